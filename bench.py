@@ -1,0 +1,184 @@
+"""Benchmark: inverted pixels/s of the batched Milne-Eddington inversion on
+one MI355X.
+
+Workload: a 1024x1024-pixel field of Fe I 6173.3356 A (5P1-5D0, g_eff 2.5)
+sampled at 30 wavelengths spaced 20 mA (-290..290 mA), Gaussian PSF of 60 mA
+FWHM, models drawn from seed 2 (B ~ U(0,2500) G, gamma, azimuth ~ U(0,180),
+vlos ~ N(0,1.5) km/s, eta0 ~ logU(1,30), Doppler width ~ U(20,50) mA,
+damping ~ U(0,0.5), S0 ~ U(0.1,0.4), S1 = 1-S0+U(-0.05,0.05)), synthesised
+by the library's own FP64 synthesis, noise sigma = 1e-3 Ic (seed 32), stored
+as float32.  Inversion: 9 free parameters, <= 50 LM iterations, the run/
+guess.mod initial model plus classical estimates per pixel.
+
+One step = one inversion pass over the whole cube, inputs resident in HBM.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--npix N]
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "p-milos_amd"))
+
+METRIC = ("pixels/s (30 lambda, 9 free, <=50 it) FP32-mixed vs FP64; "
+          "p50/p99 scan latency")
+FE6173 = 6173.3356
+TRIPLET = (2, 1, 1.0, 2, 2, 0.0)
+GUESS = np.array([20, 1100, 0.2, 0.03, 0.05, 120, 150, 0.35, 0.5, 0, 1.0])
+PEAK_FP64_TFLOPS = 78.6
+
+
+def draw_models(n, seed):
+    rng = np.random.default_rng(seed)
+    B = rng.uniform(0, 2500, n)
+    gm = rng.uniform(0, 180, n)
+    az = rng.uniform(0, 180, n)
+    vl = rng.normal(0, 1.5, n)
+    eta = np.exp(rng.uniform(0, np.log(30), n))
+    dop = rng.uniform(0.020, 0.050, n)
+    aa = rng.uniform(0, 0.5, n)
+    S0 = rng.uniform(0.1, 0.4, n)
+    S1 = 1 - S0 + rng.uniform(-0.05, 0.05, n)
+    return np.stack([eta, B, vl, dop, aa, gm, az, S0, S1, np.zeros(n), np.ones(n)], 1)
+
+
+def _cpu_worker(args):
+    """One core: synthesise its block of the workload with the oracle, add the
+    same-recipe noise, then time the oracle's inversion of the block."""
+    p0, p1, per_seed = args
+    os.environ["ORACLE_LIB"] = "liboracle_o3.so"
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    lam = O.grid(FE6173, -290, 20, 290)
+    s = O.make_setup(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam))
+    models = draw_models(p1, 2)[p0:p1]
+    spec = np.stack([O.synth(s, m) for m in models])
+    rng = np.random.default_rng(per_seed)
+    ic = (models[:, 7] + models[:, 8])[:, None, None]
+    stokes = (spec + rng.normal(0, 1, spec.shape) * 1e-3 * ic).astype(np.float32)
+    t = time.perf_counter()
+    O.invert(s, stokes, GUESS)
+    return p1 - p0, time.perf_counter() - t
+
+
+def cpu_baseline(per_core):
+    """The oracle (C restatement built -O3 -march=native) over equal pixel
+    blocks, one process per core, like pmilos.c:1181-1205.  Runs before the
+    GPU is initialised (plain fork)."""
+    cores = min(16, os.cpu_count() or 1)
+    n = per_core * cores
+    blocks = [(i * per_core, (i + 1) * per_core, 3200 + i) for i in range(cores)]
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, blocks)
+    slowest = max(r[1] for r in res)
+    return dict(value=round(n / slowest, 1), unit="pixels/s", cores=cores, kind="port",
+                sample="first %d pixels of the workload (seed-2 models, same noise "
+                       "recipe), %d processes x %d pixels, slowest block %.1f s"
+                       % (n, cores, per_core, slowest))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--npix", type=int, default=1024 * 1024)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-per-core", type=int, default=4000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding "
+              "is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_per_core)
+
+    import torch
+    import pmilos
+    from pmilos import flops
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    npix = args.npix
+    lam = pmilos.grid(FE6173, -290, 20, 290)
+    L = len(lam)
+    G = pmilos.psf_gaussian(60.0, lam)
+    ctx = pmilos.Context(FE6173, TRIPLET, lam, psf=G, guess=GUESS, noise=1e-6, miter=50)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    # synthetic cube: library FP64 synthesis, float64 noise, stored float32
+    models = torch.from_numpy(draw_models(npix, 2)).to(dev)
+    spec = torch.empty((npix, 4, L), dtype=torch.float64, device=dev)
+    ctx.synth_ptrs(models.data_ptr(), npix, spec.data_ptr(), 0, sptr)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(32)
+    ic = (models[:, 7] + models[:, 8]).view(-1, 1, 1)
+    noise = torch.randn((npix, 4, L), dtype=torch.float64, device=dev, generator=gen)
+    stokes = (spec + noise * 1e-3 * ic).to(torch.float32).contiguous()
+    del spec, noise
+    out_m = torch.empty((npix, 11), dtype=torch.float64, device=dev)
+    out_c = torch.empty(npix, dtype=torch.float32, device=dev)
+    out_i = torch.empty(npix, dtype=torch.int32, device=dev)
+    out_a = torch.empty(npix, dtype=torch.int32, device=dev)
+    torch.cuda.synchronize()
+
+    def step():
+        ctx.invert_ptrs(stokes.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
+                        out_i.data_ptr(), out_a.data_ptr(), 0, sptr)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(args.steps):
+        step()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1000.0 / args.steps
+    value = npix * args.steps / wall
+
+    iters = out_i.cpu().numpy().astype(np.float64)
+    acc = out_a.cpu().numpy().astype(np.float64)
+    fl = float(np.sum(flops.pixel_flops(iters, acc, L=L, NT=9)))
+    achieved = fl / (kern_ms * 1e-3) / 1e12
+
+    line = {
+        "metric": METRIC, "value": round(value, 1), "unit": "pixels/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (library FP64 synthesis of seeded models + seeded noise)",
+        "config": {"workload": "1024x1024 px Fe I 6173.34 A, 30 lambda @ 20 mA, PSF 60 mA, "
+                               "sigma 1e-3 Ic, 9 free, <=50 LM it, FP64 form",
+                   "npix": npix, "nlambda": L, "nterms": 9, "form": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 3),
+                     "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                     "note": "FP64 vector ALU work (the MI355X FP64 vector and matrix "
+                             "peaks are both 78.6 TFLOP/s); achieved = algorithmic "
+                             "FLOPs of the pass / pm_invert_kernel time"},
+        "kernel_ms": round(kern_ms, 3),
+        "mean_iters": round(float(iters.mean()), 3),
+        "mean_accepted": round(float(acc.mean()), 3),
+        "fallback_solves": ctx.fallback_count(),
+    }
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * pmilos.h -- C ABI of the MI355X Milne-Eddington inversion library
+ * (libpmilos_hip.so).
+ *
+ * Plain C: pointers and sizes only.  Every entry point returns PM_OK (0) or
+ * a negative error code (pm_strerror gives the text); none of them exits the
+ * process.  A context is used by one host thread at a time; its kernels run
+ * asynchronously on the stream given to the *_device entry points (NULL =
+ * the default stream), the host-pointer entry points are synchronous.
+ *
+ * Reference interfaces replaced (P-MILOS, src/):
+ *   pm_ctx_create    the per-run setup milos.c:133-263 + create_cuantic
+ *                    (create_cuantic.c:24-45) + fgauss_WL PSF (milos.c:269),
+ *                    i.e. the globals NTERMS, cuantic, G, CC that lm_mils and
+ *                    mil_sinrf read (milos.c:39-100).
+ *   pm_invert        the FITS pixel loop milos.c:958-1009 (guess copy,
+ *                    estimacionesClasicas, NaN guards, lm_mils per pixel) and
+ *                    its MPI copies pmilos.c:1256-1336; lm_mils itself is
+ *                    milosUtils.h:54-57 / milosUtils.c:466-635.
+ *   pm_synth         synthesis mode milos.c:601-602: mil_sinrf
+ *                    (mil_sinrf.c:46) + me_der (me_der.c:48).
+ *   pm_grid          wavelength axis milos.c:173-197 (readMallaGrid output).
+ *   pm_psf_gaussian  fgauss_WL, fgauss.c:79-154, as called at milos.c:269.
+ *   pm_nterms        NTERMS, milos.c:154-157.
+ */
+#ifndef PMILOS_H
+#define PMILOS_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PM_NPAR 11        /* Init_Model: eta0 B vlos dopp aa gm az S0 S1 mac alfa */
+#define PM_MAX_LAMBDA 128 /* wavelength samples per pixel supported by the kernels */
+
+enum pm_status {
+    PM_OK = 0,
+    PM_EINVAL = -22,      /* bad argument */
+    PM_ENOMEM = -12,      /* device allocation failed */
+    PM_EDEVICE = -5,      /* HIP runtime error */
+    PM_ENOTSUP = -95      /* configuration not supported by the kernels */
+};
+
+enum pm_precision {
+    PM_FP64 = 0,          /* all arithmetic in double (reference REAL=double) */
+    PM_MIXED = 1          /* FP32 spectra/RF/convolution, FP64 solve and chi2 */
+};
+
+typedef struct pm_ctx pm_ctx;
+
+/* Run configuration: the fields of ConfigControl (defines.h:223-277) and
+ * the line data that lm_mils and mil_sinrf read. */
+typedef struct {
+    int nlambda;                 /* L, 2 <= L <= PM_MAX_LAMBDA */
+    const double *lambda;        /* [L] wavelength axis in Angstrom */
+    double lambda0;              /* central wavelength of the line (wlines[1]) */
+    double sl, ll, jl;           /* lower level S, L, J (readConfig.c:104-110) */
+    double su, lu, ju;           /* upper level S, L, J */
+    const double *psf;           /* [L] PSF kernel (ConvolveWithPSF), or NULL */
+    int fix[PM_NPAR];            /* 1 = free parameter */
+    double weight[4];            /* WeightForStokes */
+    double noise;                /* per-point variance, 1/SN^2 */
+    double ilambda;              /* InitialDiagonalElement */
+    double toplim;               /* 1e-12 in the reference */
+    int miter;                   /* NumberOfCycles (> 0) */
+    double mu;                   /* heliocentric mu (ah) */
+    int logclambda;              /* Diagonal element acceleration */
+    double guess[PM_NPAR];       /* initial model (INITIAL_MODEL) */
+    int precision;               /* enum pm_precision */
+} pm_config;
+
+const char *pm_strerror(int status);
+
+/* Wavelength axis: lambda[0] = lambda0 + initial/1000, then accumulated
+ * steps of step/1000 (all in mA).  Writes min(n, cap) values, returns n. */
+int pm_grid(double lambda0, double initial_mA, double step_mA, double final_mA,
+            double *lambda, int cap);
+
+/* Gaussian PSF of the given FWHM (mA) on the axis, unit sum. */
+int pm_psf_gaussian(double fwhm_mA, const double *lambda, int nlambda, double *G);
+
+/* NTERMS for a fix vector and guess macroturbulence. */
+int pm_nterms(const int fix[PM_NPAR], double guess_mac);
+
+int  pm_ctx_create(const pm_config *cfg, pm_ctx **out);
+void pm_ctx_destroy(pm_ctx *ctx);
+int  pm_ctx_nterms(const pm_ctx *ctx);
+
+/* Synthesis (and optionally response functions) of n models.
+ * models [n][11] double; spectra [n][4][L] double; rf nullable
+ * [n][4][NT][L] double in the reference's d_spectra layout.
+ * Host pointers, synchronous. */
+int pm_synth(pm_ctx *ctx, const double *models, int n, double *spectra, double *rf);
+
+/* The same with device pointers, asynchronous on stream (hipStream_t). */
+int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectra,
+                    double *d_rf, void *stream);
+
+/* Inversion of npix pixels.  stokes [npix][4][L] float (I,Q,U,V contiguous
+ * per pixel, the forParallel layout of utilsFits.c:224,329).
+ * Outputs: models [npix][11] double (Init_Model order); chi2 [npix] float
+ * (last accepted chi2); iters [npix] int (LM trips, or the pixel index when
+ * lm_mils returns -1 for nfree == 0, as milos.c:997 leaves it); accepted
+ * nullable [npix] int (chi2-decreasing trips); fit nullable [npix][4][L] float
+ * (the last trial synthesis, milosUtils.c:569).  Host pointers, synchronous. */
+int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models,
+              float *chi2, int *iters, int *accepted, float *fit);
+
+/* The same with device pointers, asynchronous on stream (hipStream_t). */
+int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
+                     float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
+                     void *stream);
+
+/* Number of pixels whose 9x9/10x10/11x11 solve needed the eigen
+ * pseudo-inverse fallback in the last inversion on this context (diagnostic;
+ * synchronises the context's last stream). */
+int pm_last_fallback_count(pm_ctx *ctx, long long *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

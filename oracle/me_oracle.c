@@ -813,7 +813,8 @@ void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_
     int end = n;
     for (int i = n - 1; i >= 0; i--) { if (sI[i] < 0) end--; else break; }
 
-    double Ic = spectro[end - 1];                                   /* :328 */
+    /* :328; the reference reads spectro[-1] when every I sample is < 0 */
+    double Ic = (end >= 1) ? spectro[end - 1] : 0.0;
     double x = 0, y = 0, xv = 0, yv = 0;
     for (int i = begin; i < end - 1; i++) {                         /* :334-344 */
         if (sI[i] > -1 && sV[i] > -1) {

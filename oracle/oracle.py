@@ -51,7 +51,8 @@ _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 def lib():
     global _lib
     if _lib is None:
-        path = os.path.join(HERE, "liboracle.so")
+        # ORACLE_LIB=liboracle_o3.so selects the -O3 -march=native build
+        path = os.path.join(HERE, os.environ.get("ORACLE_LIB", "liboracle.so"))
         if not os.path.exists(path):
             raise RuntimeError("oracle not built: run make -C oracle")
         L = C.CDLL(path)

@@ -1,0 +1,696 @@
+// pm_device.h -- device-side building blocks of the Milne-Eddington
+// per-pixel inversion for CDNA4 (gfx950).
+//
+// Mapping: one pixel is handled by a group of W lanes of a 64-wide
+// wavefront (W = 8/16/32/64, a power of two); lane t of the group holds the
+// wavelength samples t, t+W, ... (NLPL of them).  Everything that is per
+// wavelength (Voigt, Unno-Rachkovsky, response functions) runs across lanes;
+// the PSF correlation goes through a zero-padded row buffer in LDS; sums over
+// wavelength (chi2, J^T J, gradient) are butterfly reductions inside the
+// group, after which every lane holds the same value, so the small LM
+// control state (model, damping, chi2) is kept redundantly in every lane of
+// the group and the 9x9..11x11 solve runs redundantly in every lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+#define PM_MAXC 16          // Zeeman components per group (pi / blue / red)
+#define PM_LMAX 128
+#define PM_PI 3.14159265358979323846264338327950288419716939937510
+#define PM_RR 0.5641895836          // defines.h:73 (truncated 1/sqrt(pi))
+#define PM_VLIGHT 2.99792458e+5     // defines.h:75
+#define PM_CTE4_6_13 4.6686411e-13  // defines.h:77
+
+// Per-context constants resident in device memory (read with scalar loads).
+struct PmConst {
+    int L, NT;
+    int npi, nsig;
+    int use_psf;
+    int miter, logclambda;
+    int fix[11];
+    double lambda0, geff;
+    double noise, ilambda, toplim, mu;
+    double weight[4];
+    double guess[11];
+    double nup[PM_MAXC], wep[PM_MAXC];
+    double nub[PM_MAXC], web[PM_MAXC], wer[PM_MAXC];
+    double lam[PM_LMAX];
+    double G[PM_LMAX];
+};
+
+// ------------------------------------------------------------ group helpers
+
+template <int W>
+__device__ __forceinline__ double group_sum(double v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+template <int W>
+__device__ __forceinline__ int group_sum_int(int v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    return v;
+}
+
+// Orders LDS traffic between lanes of one wavefront (groups never span
+// waves, so no workgroup barrier is needed).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// ------------------------------------------------------------ Voigt
+
+// H and F (= Im/2) of the 7/8-term complex rational function P(z)/Q(z),
+// z = a - i v, Horner order of fvoigt.c:36-61, coefficients fvoigt.c:20-26.
+template <typename T>
+__device__ __forceinline__ void voigt(T a, T v, T &H, T &F)
+{
+    const T A0 = T(122.607931777104326), A1 = T(214.382388694706425),
+            A2 = T(181.928533092181549), A3 = T(93.155580458138441),
+            A4 = T(30.180142196210589), A5 = T(5.912626209773153),
+            A6 = T(0.564189583562615);
+    const T B0 = T(122.60793177387535), B1 = T(352.730625110963558),
+            B2 = T(457.334478783897737), B3 = T(348.703917719495792),
+            B4 = T(170.354001821091472), B5 = T(53.992906912940207),
+            B6 = T(10.479857114260399);
+    const T zr = a, zi = -v;
+    // numerator: a6, then (n*z + a_j)
+    T nr = A6, ni = T(0);
+#define PM_HORNER(CR, CI, K)                    \
+    {                                           \
+        T tr = CR * zr - CI * zi + K;           \
+        T ti = CR * zi + CI * zr;               \
+        CR = tr; CI = ti;                       \
+    }
+    PM_HORNER(nr, ni, A5) PM_HORNER(nr, ni, A4) PM_HORNER(nr, ni, A3)
+    PM_HORNER(nr, ni, A2) PM_HORNER(nr, ni, A1) PM_HORNER(nr, ni, A0)
+    // denominator: z + b6, then (d*z + b_j)
+    T dr = zr + B6, di = zi;
+    PM_HORNER(dr, di, B5) PM_HORNER(dr, di, B4) PM_HORNER(dr, di, B3)
+    PM_HORNER(dr, di, B2) PM_HORNER(dr, di, B1) PM_HORNER(dr, di, B0)
+#undef PM_HORNER
+    // w = n / d (Smith's scaling keeps the FP32 path in range)
+    T wr, wi;
+    if (fabs(dr) >= fabs(di)) {
+        T r = di / dr, den = dr + di * r;
+        T inv = T(1) / den;
+        wr = (nr + ni * r) * inv;
+        wi = (ni - nr * r) * inv;
+    } else {
+        T r = dr / di, den = dr * r + di;
+        T inv = T(1) / den;
+        wr = (nr * r + ni) * inv;
+        wi = (ni * r - nr) * inv;
+    }
+    H = wr;
+    F = wi * T(0.5);
+}
+
+// ------------------------------------------------------------ synthesis
+
+// Angle terms of mil_sinrf.c:84-103 (shared by synthesis and RFs).
+template <typename T>
+struct Geom {
+    T sin_gm, cosi, sinis, cosis, cosis_2, sina, cosa;
+    T sinda, cosda, sindi, cosdi, sinis_cosa, sinis_sina;
+    T E0, LD, A, MF, B0, B1, ulos, parcial;
+};
+
+template <typename T>
+__device__ __forceinline__ void make_geom(const PmConst *__restrict__ c, const double *m,
+                                          Geom<T> &g)
+{
+    const double CC = PM_PI / 180.0, CC_2 = CC * 2;      // milos.c:232-233
+    double GM = m[5] * CC, AZI = m[6] * CC;
+    double sgm = sin(GM), cgm = cos(GM);
+    double azi_2 = 2 * AZI;
+    double sa = sin(azi_2), ca = cos(azi_2);
+    double sinis = sgm * sgm, cosis = cgm * cgm;
+    g.sin_gm = T(sgm); g.cosi = T(cgm);
+    g.sinis = T(sinis); g.cosis = T(cosis); g.cosis_2 = T((1 + cosis) / 2);
+    g.sina = T(sa); g.cosa = T(ca);
+    g.sinda = T(ca * CC_2); g.cosda = T(-sa * CC_2);
+    g.sindi = T(cgm * sgm * CC_2); g.cosdi = T(-sgm * CC);
+    g.sinis_cosa = T(sinis * ca); g.sinis_sina = T(sinis * sa);
+    g.E0 = T(m[0]);                                        // FO = 1
+    g.LD = T(m[3]); g.A = T(m[4]); g.MF = T(m[1]);
+    g.B0 = T(m[7]); g.B1 = T(-(m[8] * c->mu));
+    g.ulos = T((m[2] * c->lambda0) / (PM_VLIGHT * m[3]));      // mil_sinrf.c:120
+    g.parcial = T(((c->lambda0 * c->lambda0) / m[3]) * PM_CTE4_6_13);  // :153
+}
+
+// Per-wavelength state of one synthesis kept for the response functions:
+// the component sums and the last component of each group (me_der.c keeps
+// only the last component's derivative, me_der.c:569-600).
+template <typename T>
+struct Samp {
+    T fi[3], shi[3];          // p, b, r
+    T uu[3], H[3], F[3];      // last component of each group
+};
+
+// Components and profile sums at one wavelength, mil_sinrf.c:122-183,418-449.
+template <typename T>
+__device__ __forceinline__ void profiles(const PmConst *__restrict__ c, const Geom<T> &g,
+                                         T lam, Samp<T> &s)
+{
+    T u = ((lam - T(c->lambda0)) / g.LD) - g.ulos;
+    // pi components
+    {
+        T fi = 0, shi = 0;
+        for (int k = 0; k < c->npi; k++) {
+            T w = T(c->wep[k]);
+            T uu = u - (g.parcial * T(c->nup[k])) * g.MF;
+            T H, F;
+            voigt(g.A, uu, H, F);
+            fi = fi + w * H;
+            shi = shi + w * F * T(2);
+            s.uu[0] = uu; s.H[0] = H; s.F[0] = F;
+        }
+        s.fi[0] = fi; s.shi[0] = shi;
+    }
+    // blue sigma
+    {
+        T fi = 0, shi = 0;
+        for (int k = 0; k < c->nsig; k++) {
+            T w = T(c->web[k]);
+            T uu = u - (g.parcial * T(c->nub[k])) * g.MF;
+            T H, F;
+            voigt(g.A, uu, H, F);
+            fi = fi + w * H;
+            shi = shi + w * F * T(2);
+            s.uu[1] = uu; s.H[1] = H; s.F[1] = F;
+        }
+        s.fi[1] = fi; s.shi[1] = shi;
+    }
+    // red sigma: shifts mirrored from the blue ones (mil_sinrf.c:164-166)
+    {
+        T fi = 0, shi = 0;
+        for (int k = 0; k < c->nsig; k++) {
+            T w = T(c->wer[k]);
+            T nur = -(g.parcial * T(c->nub[c->nsig - 1 - k]));
+            T uu = u - nur * g.MF;
+            T H, F;
+            voigt(g.A, uu, H, F);
+            fi = fi + w * H;
+            shi = shi + w * F * T(2);
+            s.uu[2] = uu; s.H[2] = H; s.F[2] = F;
+        }
+        s.fi[2] = fi; s.shi[2] = shi;
+    }
+}
+
+// Absorption matrix and Unno-Rachkovsky quantities, mil_sinrf.c:188-266.
+template <typename T>
+struct UR {
+    T etai, etaq, etau, etav, rhoq, rhou, rhov;
+    T etai_2, gp1, gp2, gp3, dt, dti, g4r, g5r, g6r;
+    T parcial1, parcial2, parcial3;
+};
+
+template <typename T>
+__device__ __forceinline__ void unno(const Geom<T> &g, const Samp<T> &s, UR<T> &r)
+{
+    T E0_2 = g.E0 / T(2);
+    r.parcial1 = s.fi[1] + s.fi[2];
+    r.parcial2 = (E0_2) * (s.fi[0] - (r.parcial1) / T(2));
+    r.parcial3 = (E0_2) * (s.shi[0] - (s.shi[1] + s.shi[2]) / T(2));
+    T cosi_E0_2 = E0_2 * g.cosi;
+    T etain = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2));
+    r.etai = T(1) + etain;
+    r.etaq = (r.parcial2 * g.sinis_cosa);
+    r.etau = (r.parcial2 * g.sinis_sina);
+    r.etav = (s.fi[2] - s.fi[1]) * cosi_E0_2;
+    r.rhoq = (r.parcial3 * g.sinis_cosa);
+    r.rhou = (r.parcial3 * g.sinis_sina);
+    r.rhov = (s.shi[2] - s.shi[1]) * cosi_E0_2;
+    T ei = r.etai, eq = r.etaq, eu = r.etau, ev = r.etav;
+    T rq = r.rhoq, ru = r.rhou, rv = r.rhov;
+    T ei2 = ei * ei;
+    T auxq = rq * rq, auxu = ru * ru, auxv = rv * rv;
+    r.etai_2 = ei2;
+    r.gp1 = ei2 - eq * eq - eu * eu - ev * ev + auxq + auxu + auxv;
+    r.gp3 = ei2 + auxq + auxu + auxv;
+    r.gp2 = eq * rq + eu * ru + ev * rv;
+    r.dt = ei2 * r.gp1 - r.gp2 * r.gp2;
+    r.dti = T(1) / r.dt;
+    T gp4 = ei2 * eq + ei * (ev * ru - eu * rv);
+    T gp5 = ei2 * eu + ei * (eq * rv - ev * rq);
+    T gp6 = (ei2) * ev + ei * (eu * rq - eq * ru);
+    r.g4r = gp4 + rq * r.gp2;
+    r.g5r = gp5 + ru * r.gp2;
+    r.g6r = gp6 + rv * r.gp2;
+}
+
+// Emergent Stokes vector before convolution, mil_sinrf.c:268-278.
+template <typename T>
+__device__ __forceinline__ void stokes(const Geom<T> &g, const UR<T> &r, T S[4])
+{
+    T dtiaux = r.dti * (g.B1);
+    S[0] = g.B0 - dtiaux * r.etai * r.gp3;
+    S[1] = (dtiaux * (r.g4r));
+    S[2] = (dtiaux * (r.g5r));
+    S[3] = (dtiaux * (r.g6r));
+}
+
+// Response functions of parameters 0..8 at one wavelength before
+// convolution, me_der.c:93-312 (d_spectra rows eta0 B vlos dopp aa gm az S0
+// S1).  D[s][p].
+template <typename T, int NT>
+__device__ __forceinline__ void response(const PmConst *__restrict__ c, const Geom<T> &g,
+                                         const Samp<T> &s, const UR<T> &r, T D[4][NT])
+{
+    const T LD = g.LD, A = g.A, E0 = g.E0;
+    // per-group derivatives of the profiles w.r.t. B, vlos, dopp, a
+    // (funcionComponentFor, me_der.c:536-609; weights of the last component)
+    T dfi[3][4], dshi[3][4];
+    const T auxCte = T(-c->lambda0) / (T(PM_VLIGHT) * LD);
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        T w, nux;
+        if (q == 0) {
+            w = T(c->wep[c->npi - 1]);
+            nux = g.parcial * T(c->nup[c->npi - 1]);
+        } else if (q == 1) {
+            w = T(c->web[c->nsig - 1]);
+            nux = g.parcial * T(c->nub[c->nsig - 1]);
+        } else {
+            w = T(c->wer[c->nsig - 1]);
+            nux = -(g.parcial * T(c->nub[0]));
+        }
+        T dH_u = ((T(4) * A * s.F[q]) - (T(2) * s.uu[q] * s.H[q])) * w;
+        T dF_u = (T(PM_RR) - A * s.H[q] - T(2) * s.uu[q] * s.F[q]) * w;
+        T uun = -s.uu[q] / LD;
+        dfi[q][0] = dH_u * (-nux);
+        dfi[q][1] = dH_u * auxCte;
+        dfi[q][2] = (dH_u * uun);
+        dfi[q][3] = (T(-2) * dF_u);
+        dshi[q][0] = (dF_u) * (-nux);
+        dshi[q][1] = dF_u * auxCte;
+        dshi[q][2] = (dF_u * uun);
+        dshi[q][3] = (dH_u / T(2));
+    }
+    // derivatives of eta / rho (me_der.c:93-199)
+    T dei[7], deq[7], deu[7], dev[7], drq[7], dru[7], drv[7];
+    T E0_2 = E0 / T(2);
+    T cosi_E0_2 = E0_2 * g.cosi;
+    dei[0] = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2)) / E0;
+    deq[0] = (r.parcial2 * g.sinis_cosa) / E0;
+    deu[0] = (r.parcial2 * g.sinis_sina) / E0;
+    dev[0] = ((s.fi[2] - s.fi[1]) * cosi_E0_2) / E0;
+    drq[0] = (r.parcial3 * g.sinis_cosa) / E0;
+    dru[0] = (r.parcial3 * g.sinis_sina) / E0;
+    drv[0] = ((s.shi[2] - s.shi[1]) * cosi_E0_2) / E0;
+    T sc = E0 * g.sinis_cosa / T(2);
+    T ss = E0 * g.sinis_sina / T(2);
+    T sindi_cosa = g.sindi * g.cosa, sindi_sina = g.sindi * g.sina;
+    T cosdi_E0_2 = (E0_2) * g.cosdi;
+    T cosi_2_E0 = E0 * g.cosi / T(2);
+    T cosis_2_E0_2 = g.cosis_2 * E0_2;
+    T sinis_E0_2 = g.sinis * E0_2;
+#pragma unroll
+    for (int j = 1; j < 5; j++) {
+        T dfisum = dfi[1][j - 1] + dfi[2][j - 1];
+        dei[j] = (dfi[0][j - 1] * sinis_E0_2 + dfisum * cosis_2_E0_2);
+        T aux = dfi[0][j - 1] - dfisum / T(2);
+        deq[j] = (aux) * sc;
+        deu[j] = (aux) * ss;
+        dev[j] = (dfi[2][j - 1] - dfi[1][j - 1]) * cosi_2_E0;
+        T auxr = dshi[0][j - 1] - (dshi[1][j - 1] + dshi[2][j - 1]) / T(2);
+        drq[j] = (auxr) * sc;
+        dru[j] = (auxr) * ss;
+        drv[j] = ((dshi[2][j - 1] - dshi[1][j - 1])) * cosi_2_E0;
+    }
+    T cosi_cosdi = g.cosi * g.cosdi * E0_2;
+    T sindi_E0_2 = g.sindi * E0_2;
+    dei[5] = s.fi[0] * sindi_E0_2 + (r.parcial1) * cosi_cosdi;
+    deq[5] = r.parcial2 * sindi_cosa;
+    deu[5] = r.parcial2 * sindi_sina;
+    dev[5] = (s.fi[2] - s.fi[1]) * cosdi_E0_2;
+    drq[5] = r.parcial3 * sindi_cosa;
+    dru[5] = r.parcial3 * sindi_sina;
+    drv[5] = (s.shi[2] - s.shi[1]) * cosdi_E0_2;
+    T sinis_cosda = g.sinis * g.cosda, sinis_sinda = g.sinis * g.sinda;
+    dei[6] = 0;
+    deq[6] = r.parcial2 * sinis_cosda;
+    deu[6] = r.parcial2 * sinis_sinda;
+    dev[6] = 0;
+    drq[6] = r.parcial3 * sinis_cosda;
+    dru[6] = r.parcial3 * sinis_sinda;
+    drv[6] = 0;
+
+    // chain rule through the Unno-Rachkovsky solution (me_der.c:207-277)
+    const T ei = r.etai, eq = r.etaq, eu = r.etau, ev = r.etav;
+    const T rq = r.rhoq, ru = r.rhou, rv = r.rhov;
+    const T ei2 = r.etai_2, gp1 = r.gp1, gp2 = r.gp2, gp3 = r.gp3, dt = r.dt;
+    const T dtaux = (g.B1) / (dt * dt);
+    const T etai_gp3 = ei * gp3;
+    const T aux2 = T(2) * ei;
+    const T ext1 = aux2 * eq + ev * ru - eu * rv;
+    const T ext2 = aux2 * eu + eq * rv - ev * rq;
+    const T ext3 = aux2 * ev + eu * rq - eq * ru;
+    const T ext4 = aux2 * gp1;
+#pragma unroll
+    for (int il = 0; il < 7; il++) {
+        T a_ei = dei[il], a_eq = deq[il], a_eu = deu[il], a_ev = dev[il];
+        T a_rq = drq[il], a_ru = dru[il], a_rv = drv[il];
+        T dgp1 = T(2) * (ei * a_ei - eq * a_eq - eu * a_eu - ev * a_ev + rq * a_rq +
+                         ru * a_ru + rv * a_rv);
+        T dgp2 = rq * a_eq + eq * a_rq + ru * a_eu + eu * a_ru + rv * a_ev + ev * a_rv;
+        T d_dt = ext4 * a_ei + ei2 * dgp1 - T(2) * gp2 * dgp2;
+        T dgp3 = T(2) * (ei * a_ei + rq * a_rq + ru * a_ru + rv * a_rv);
+        D[0][il] = -(((a_ei * gp3 + ei * dgp3) * dt - d_dt * etai_gp3) * (dtaux));
+        T dgp4 = a_ei * (ext1) + (ei2) * a_eq + ei * (ru * a_ev + ev * a_ru - rv * a_eu - eu * a_rv);
+        D[1][il] = ((dgp4 + a_rq * gp2 + rq * dgp2) * dt - d_dt * (r.g4r)) * (dtaux);
+        T dgp5 = a_ei * (ext2) + (ei2) * a_eu + ei * (rv * a_eq + eq * a_rv - rq * a_ev - ev * a_rq);
+        D[2][il] = ((dgp5 + a_ru * gp2 + ru * dgp2) * dt - d_dt * (r.g5r)) * (dtaux);
+        T dgp6 = a_ei * (ext3) + (ei2) * a_ev + ei * (rq * a_eu + eu * a_rq - ru * a_eq - eq * a_ru);
+        D[3][il] = ((dgp6 + a_rv * gp2 + rv * dgp2) * dt - d_dt * (r.g6r)) * (dtaux);
+    }
+    // S1 (me_der.c:281-298), S0 (:301-306), azimuth on I and V (:309-312)
+    T dtin = -(r.dti * T(c->mu));
+    D[0][8] = -dtin * etai_gp3;
+    D[1][8] = dtin * (r.g4r);
+    D[2][8] = dtin * (r.g5r);
+    D[3][8] = dtin * (r.g6r);
+    D[0][7] = T(1); D[1][7] = 0; D[2][7] = 0; D[3][7] = 0;
+    D[0][6] = 0; D[3][6] = 0;
+#pragma unroll
+    for (int p = 9; p < NT; p++) { D[0][p] = 0; D[1][p] = 0; D[2][p] = 0; D[3][p] = 0; }
+}
+
+// ------------------------------------------------------------ model update
+
+// AplicaDelta, milosUtils.c:47-115.
+__device__ __forceinline__ void aplica_delta(const PmConst *__restrict__ c, int NT,
+                                             const double *cur, double *delta, double *out)
+{
+    const int *f = c->fix;
+    if (f[0]) out[0] = cur[0] - delta[0];
+    if (f[1]) {
+        if (delta[1] < -300) delta[1] = -300;
+        else if (delta[1] > 300) delta[1] = 300;
+        out[1] = cur[1] - delta[1];
+    }
+    if (f[2]) out[2] = cur[2] - delta[2];
+    if (f[3]) out[3] = cur[3] - delta[3];
+    if (f[4]) out[4] = cur[4] - delta[4];
+    if (f[5]) {
+        if (delta[5] < -30) delta[5] = -30;
+        else if (delta[5] > 30) delta[5] = 30;
+        out[5] = cur[5] - delta[5];
+    }
+    if (f[6]) {
+        if (delta[6] < -30) delta[6] = -30;
+        else if (delta[6] > 30) delta[6] = 30;
+        out[6] = cur[6] - delta[6];
+    }
+    if (f[7]) out[7] = cur[7] - delta[7];
+    if (f[8]) out[8] = cur[8] - delta[8];
+    if (NT > 9 && f[9]) out[9] = cur[9] - delta[9];
+    if (f[10]) {
+        if (NT == 11) out[10] = cur[10] - delta[10];
+        else if (NT > 9) out[10] = cur[10] - delta[9];
+    }
+}
+
+// check, milosUtils.c:118-196.
+__device__ __forceinline__ void check_model(double *m)
+{
+    if (m[1] < 0) { m[1] = -(m[1]); m[5] = 180.0 - (m[5]); }
+    if (m[1] > 5000) m[1] = 5000;
+    if (m[5] < 0) m[5] = -(m[5]);
+    if (m[5] > 180) m[5] = 360.0 - m[5];
+    if (m[6] < 0) m[6] = 180 + (m[6]);
+    if (m[6] > 180) m[6] = m[6] - 180.0;
+    if (m[0] < 1) m[0] = 1;
+    if (m[0] > 2500) m[0] = 2500;
+    if (m[2] < (-20)) m[2] = (-20);
+    if (m[2] > 20) m[2] = 20;
+    if (m[3] < 0.0001) m[3] = 0.0001;
+    if (m[3] > 0.6) m[3] = 0.6;
+    if (m[4] < 0.0001) m[4] = 0.0001;
+    if (m[4] > 10.0) m[4] = 10.0;
+    if (m[7] < 0.0001) m[7] = 0.0001;
+    if (m[7] > 2.00) m[7] = 2.00;
+    if (m[8] < 0.0001) m[8] = 0.0001;
+    if (m[8] > 2.00) m[8] = 2.00;
+    if (m[9] < 0) m[9] = 0;
+    if (m[9] > 4) m[9] = 4;
+    if (m[10] < 0) m[10] = 0.0;
+    if (m[10] > 1.0) m[10] = 1.0;
+}
+
+// ------------------------------------------------------------ solve
+
+// delta = A^+ beta for the damped normal matrix.  The reference forms the
+// eigen pseudo-inverse with an absolute cutoff |w| > 1e-12
+// (milosUtils.c:237-271).  For an exactly zero row (a fixed parameter) the
+// eigenvalue is 0 and that component is 0; on the remaining block, when an
+// LDL^T factorisation succeeds and 1/trace(A^-1) (a lower bound of the
+// smallest eigenvalue) exceeds 1e-9, every eigenvalue is far above the cutoff
+// and the pseudo-inverse is the inverse.  Otherwise returns false and the
+// caller uses the eigen pseudo-inverse itself.
+template <int N>
+__device__ __forceinline__ bool ldl_solve(const double *Ap /* packed lower, N(N+1)/2 */,
+                                          const double *b, double *x)
+{
+    constexpr int P = N * (N + 1) / 2;
+    double Lm[P];
+    double d[N];
+    bool act[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) act[i] = Ap[i * (i + 1) / 2 + i] != 0.0;
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        const int jj = j * (j + 1) / 2;
+        if (!act[j]) {
+            d[j] = 1.0;
+#pragma unroll
+            for (int k = 0; k < j; k++) Lm[jj + k] = 0.0;
+            Lm[jj + j] = 1.0;
+            continue;
+        }
+        double dj = Ap[jj + j];
+#pragma unroll
+        for (int k = 0; k < j; k++) {
+            double l = Lm[jj + k];
+            dj -= l * l * d[k];
+        }
+        if (!(dj > 0.0)) return false;
+        d[j] = dj;
+        Lm[jj + j] = 1.0;
+#pragma unroll
+        for (int i = j + 1; i < N; i++) {
+            const int ii = i * (i + 1) / 2;
+            double v = act[i] ? Ap[ii + j] : 0.0;
+#pragma unroll
+            for (int k = 0; k < j; k++) v -= Lm[ii + k] * Lm[jj + k] * d[k];
+            // stored transposed-free: row i, column j
+            Lm[ii + j] = act[i] ? v / dj : 0.0;
+        }
+    }
+    // trace(A^-1) = sum_k (1/d_k) sum_i (L^-1)_{ki}^2 over the active block
+    double Mi[P];
+    double tr = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int ii = i * (i + 1) / 2;
+        Mi[ii + i] = 1.0;
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            double v = 0.0;
+#pragma unroll
+            for (int k = j; k < i; k++) v -= Lm[ii + k] * Mi[k * (k + 1) / 2 + j];
+            Mi[ii + j] = v;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        if (!act[k]) continue;
+        double s = 0.0;
+        const int kk = k * (k + 1) / 2;
+#pragma unroll
+        for (int i = 0; i <= k; i++) s += Mi[kk + i] * Mi[kk + i];
+        tr += s / d[k];
+    }
+    if (!(tr < 1e9)) return false;
+    // forward, diagonal, backward
+    double y[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const int ii = i * (i + 1) / 2;
+        double v = act[i] ? b[i] : 0.0;
+#pragma unroll
+        for (int k = 0; k < i; k++) v -= Lm[ii + k] * y[k];
+        y[i] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) y[i] /= d[i];
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        double v = y[i];
+#pragma unroll
+        for (int k = i + 1; k < N; k++) v -= Lm[k * (k + 1) / 2 + i] * x[k];
+        x[i] = act[i] ? v : 0.0;
+    }
+    return true;
+}
+
+// Eigen pseudo-inverse by cyclic Jacobi rotations on a matrix kept in LDS,
+// the lanes of the group sharing the row/column updates.  Same rotation
+// formula, sweep order and stopping rule as the CPU restatement.  A: N*N in
+// LDS (overwritten), V: N*N in LDS; beta, delta in registers (uniform).
+template <int W, int N>
+__device__ void jacobi_pinv(double *A, double *V, int t, const double *beta, double *delta)
+{
+    for (int i = t; i < N * N; i += W) V[i] = ((i / N) == (i % N)) ? 1.0 : 0.0;
+    wave_sync();
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0, diag = 0;
+        for (int i = 0; i < N; i++) {
+            diag += A[i * N + i] * A[i * N + i];
+            for (int j = i + 1; j < N; j++) off += A[i * N + j] * A[i * N + j];
+        }
+        if (off == 0.0 || off <= 1e-36 * diag) break;
+        for (int p = 0; p < N - 1; p++) {
+            for (int q = p + 1; q < N; q++) {
+                double apq = A[p * N + q];
+                if (apq == 0.0) continue;
+                double app = A[p * N + p], aqq = A[q * N + q];
+                double theta = (aqq - app) / (2.0 * apq);
+                double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
+                wave_sync();
+                for (int k = t; k < N; k += W) {
+                    double akp = A[k * N + p], akq = A[k * N + q];
+                    A[k * N + p] = cs * akp - sn * akq;
+                    A[k * N + q] = sn * akp + cs * akq;
+                }
+                wave_sync();
+                for (int k = t; k < N; k += W) {
+                    double apk = A[p * N + k], aqk = A[q * N + k];
+                    A[p * N + k] = cs * apk - sn * aqk;
+                    A[q * N + k] = sn * apk + cs * aqk;
+                }
+                wave_sync();
+                if (t == 0) { A[p * N + q] = 0.0; A[q * N + p] = 0.0; }
+                for (int k = t; k < N; k += W) {
+                    double vkp = V[k * N + p], vkq = V[k * N + q];
+                    V[k * N + p] = cs * vkp - sn * vkq;
+                    V[k * N + q] = sn * vkp + cs * vkq;
+                }
+                wave_sync();
+            }
+        }
+    }
+    double aux[N];
+    for (int j = 0; j < N; j++) {
+        double s = 0;
+        for (int i = 0; i < N; i++) s += beta[i] * V[i * N + j];
+        double w = A[j * N + j];
+        aux[j] = s * ((fabs(w) > 1e-12) ? (1 / w) : 0.0);
+    }
+    for (int i = 0; i < N; i++) {
+        double s = 0;
+        for (int j = 0; j < N; j++) s += V[i * N + j] * aux[j];
+        delta[i] = s;
+    }
+    wave_sync();
+}
+
+// ------------------------------------------------------------ classical estimates
+
+// estimacionesClasicas(forInitialUse = 1), milosUtils.c:287-452, on the
+// observed profile staged in LDS (each lane runs it redundantly, in the
+// reference's summation order).  Writes B, vlos, gm, az of m.
+__device__ inline void classical(const PmConst *__restrict__ c, const float *spec, double *m)
+{
+    const int n = c->L;
+    const double l0 = c->lambda0;
+    const float *sI = spec, *sQ = spec + n, *sU = spec + 2 * n, *sV = spec + 3 * n;
+    int begin = 0;
+    for (int i = 0; i < n; i++) { if (sI[i] < 0) begin++; else break; }
+    int end = n;
+    for (int i = n - 1; i >= 0; i--) { if (sI[i] < 0) end--; else break; }
+    double Ic = (end >= 1) ? (double)spec[end - 1] : 0.0;
+    double x = 0, y = 0, xv = 0, yv = 0;
+    for (int i = begin; i < end - 1; i++) {
+        if (sI[i] > -1 && sV[i] > -1) {
+            double aux = (Ic - ((double)sI[i] + (double)sV[i]));
+            double auxv = (Ic - (double)sI[i]);
+            double dl = c->lam[i] - l0;
+            x += (aux * dl);
+            xv += (auxv * dl);
+            y += aux;
+            yv += auxv;
+        }
+    }
+    double lplus = (fabs(y) > 1e-15) ? x / y : 0;
+    x = 0; y = 0;
+    for (int i = begin; i < end - 1; i++) {
+        if (sI[i] > -1 && sV[i] > -1) {
+            double aux = (Ic - ((double)sI[i] - (double)sV[i]));
+            x += (aux * (c->lam[i] - l0));
+            y += aux;
+        }
+    }
+    double lminus = (fabs(y) > 1e-15) ? x / y : 0;
+    double C = (PM_CTE4_6_13 * (l0 * l0) * c->geff);
+    double Blos = (1 / C) * ((lplus - lminus) / 2);
+    double Vlos = (PM_VLIGHT / (l0)) * ((xv / yv) / 2);
+    x = 0; y = 0;
+    for (int i = begin; i < end - 1; i++) {
+        if (sQ[i] > -1 && sU[i] > -1 && sV[i] > -1) {
+            // float arithmetic as in the reference (float * float)
+            float qu = __fadd_rn(__fmul_rn(sQ[i], sQ[i]), __fmul_rn(sU[i], sU[i]));
+            double Lp = fabs(sqrt((double)qu));
+            double mm = fabs((4 * (c->lam[i] - l0) * Lp));
+            double av = fabs((double)sV[i]);
+            x = x + av * mm;
+            y = y + av * av;
+        }
+    }
+    y = y * fabs((3 * C * Blos));
+    double tan_gamma = fabs(sqrt(x / y));
+    double gamma_rad = atan(tan_gamma);
+    double gamma = gamma_rad * (180 / PM_PI);
+    if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (PM_PI / 180);
+    if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (PM_PI / 180);
+    if (Blos < 0) gamma = (180) - gamma;
+    double su = 0.0, sq = 0.0;
+    for (int i = 0; i < n; i++) {
+        if (sU[i] > -1 && sQ[i] > -1) {
+            if (fabs((double)sU[i]) > 0.0001 || fabs((double)sQ[i]) > 0.0001) {
+                su += sU[i];
+                sq += sQ[i];
+            }
+        }
+    }
+    double tan2phi = su / sq;
+    double phi = (atan(tan2phi) * 180 / PM_PI) / 2;
+    if (su > 0 && sq > 0) phi = phi;
+    else if (su < 0 && sq > 0) phi = phi + 180;
+    else if (su < 0 && sq < 0) phi = phi + 90;
+    else if (su > 0 && sq < 0) phi = phi + 90;
+    double B_aux = fabs(Blos / cos(gamma_rad));
+    if (Vlos < (-4)) Vlos = -4;
+    if (Vlos > (4)) Vlos = 4;
+    m[1] = (B_aux > 4000 ? 4000 : B_aux);
+    m[2] = Vlos;
+    m[5] = gamma;
+    m[6] = phi;
+    // NaN guards of the pixel loop, milos.c:976-983
+    if (isnan(m[1])) m[1] = 1;
+    if (isnan(m[2])) m[2] = 1e-3;
+    if (isnan(m[5])) m[5] = 1;
+    if (isnan(m[6])) m[6] = 1;
+}

@@ -1,0 +1,822 @@
+// pm_kernels.hip -- Milne-Eddington inversion kernels for MI355X (gfx950)
+// and the C ABI declared in include/pmilos.h.
+//
+// Kernels
+//   pm_invert_kernel  persistent: each group of W lanes pulls pixels from
+//                     per-XCD work counters and runs the whole per-pixel path
+//                     (classical estimates -> LM loop of lm_mils) on one pixel
+//                     at a time, so a converged pixel frees its lanes for the
+//                     next one immediately (no padding of short pixels to the
+//                     longest one).
+//   pm_synth_kernel   synthesis (+ response functions) of given models.
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "pm_device.h"
+#include "../../include/pmilos.h"
+
+#define PM_BLOCK 256
+#define PM_CHUNK 4           // pixels taken per work-counter increment
+#define PM_NSLICE 8          // one work counter per XCD
+#define PM_CSTRIDE 32        // ints between counters (separate cache lines)
+
+// ------------------------------------------------------------ LDS layout
+
+template <typename T, int W, int NLPL, int NT>
+struct Lds {
+    static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
+    static constexpr int PAD_BYTES = 4 * PADW * (int)sizeof(T);
+    static constexpr int EIG_BYTES = 2 * NT * NT * 8;
+    static constexpr int U_BYTES =
+        ((PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES) + 15) / 16 * 16;
+    static constexpr int OBS_BYTES = (4 * W * NLPL * 4 + 15) / 16 * 16;
+    static constexpr int NP = NT * (NT + 1) / 2;
+    static constexpr int AB_BYTES = ((NP + NT) * 8 + 15) / 16 * 16;
+    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES;
+    static constexpr int GPB = PM_BLOCK / W;
+    static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
+};
+
+// ------------------------------------------------------------ convolution
+
+// Zero-padded "same" correlation of the four Stokes rows x[s][slot] with the
+// PSF, in place (convolution.c:54-78); when ic_row0 the I row uses the
+// continuum trick of convolution.c:90-113 with Ic = max(first, last) of the
+// row (mil_sinrf.c:390-395, me_der.c:478-483).  pad: 4 rows of PADW values,
+// zero outside [L/2, L/2 + L).  Accumulation in double (convolution.c:68).
+template <typename T, int W, int NLPL>
+__device__ __forceinline__ void conv4(const PmConst *__restrict__ c, T *pad, int t, int L,
+                                      T x[4][NLPL], bool ic_row0)
+{
+    constexpr int PADW = 2 * W * NLPL;
+    const int half = L / 2;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L) {
+#pragma unroll
+            for (int r = 0; r < 4; r++) pad[r * PADW + half + lam] = x[r][k];
+        }
+    }
+    wave_sync();
+    T ic = 0;
+    if (ic_row0) {
+        T a = pad[half], b = pad[half + L - 1];
+        ic = (a > b) ? a : b;
+        wave_sync();
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            int lam = t + k * W;
+            if (lam < L) pad[half + lam] = ic - x[0][k];
+        }
+        wave_sync();
+    }
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L) {
+            double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            const T *p0 = pad + lam;
+            for (int j = 0; j < L; j++) {
+                double gj = c->G[j];
+                acc0 += gj * (double)p0[j];
+                acc1 += gj * (double)p0[PADW + j];
+                acc2 += gj * (double)p0[2 * PADW + j];
+                acc3 += gj * (double)p0[3 * PADW + j];
+            }
+            x[0][k] = ic_row0 ? T((double)ic - acc0) : T(acc0);
+            x[1][k] = T(acc1);
+            x[2][k] = T(acc2);
+            x[3][k] = T(acc3);
+        }
+    }
+    wave_sync();
+}
+
+// Zero the response-function row of a fixed parameter,
+// FijaACeroDerivadasNoNecesarias, milosUtils.c:199-227.
+__device__ __forceinline__ bool row_is_fixed(const PmConst *__restrict__ c, int NT, int p)
+{
+    if (NT == 9 || NT == 11) return c->fix[p] == 0;
+    if (p < 9) return c->fix[p] == 0;
+    return c->fix[9] == 0 && c->fix[10] == 0;
+}
+
+// Synthesis of model m for this lane's wavelength slots: fills the sample
+// state (for the RFs) and the emergent, convolved Stokes vector S.
+template <typename T, int W, int NLPL>
+__device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const double *m,
+                                            T *pad, int t, int L, Geom<T> &g,
+                                            Samp<T> sp[NLPL], T S[4][NLPL])
+{
+    make_geom<T>(c, m, g);
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        int li = lam < L ? lam : L - 1;
+        profiles<T>(c, g, T(c->lam[li]), sp[k]);
+        UR<T> r;
+        unno<T>(g, sp[k], r);
+        T s4[4];
+        stokes<T>(g, r, s4);
+#pragma unroll
+        for (int s = 0; s < 4; s++) S[s][k] = s4[s];
+    }
+    if (c->use_psf) conv4<T, W, NLPL>(c, pad, t, L, S, true);   // mil_sinrf.c:387-402
+}
+
+// Response functions of the synthesis held in sp, convolved like the
+// spectra (me_der.c:471-495, S0 row never), one parameter at a time.
+template <typename T, int W, int NLPL, int NT>
+__device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Geom<T> &g,
+                                         const Samp<T> sp[NLPL], T *pad, int t, int L,
+                                         T D[NLPL][4][NT])
+{
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        UR<T> r;
+        unno<T>(g, sp[k], r);
+        response<T, NT>(c, g, sp[k], r, D[k]);
+    }
+    if (c->use_psf) {
+#pragma unroll
+        for (int p = 0; p < NT; p++) {
+            if (p == 7) continue;
+            T x[4][NLPL];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) x[s][k] = D[k][s][p];
+            conv4<T, W, NLPL>(c, pad, t, L, x, true);
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) D[k][s][p] = x[s][k];
+        }
+    }
+}
+
+// ------------------------------------------------------------ work queue
+
+struct InvArgs {
+    const PmConst *c;
+    const float *stokes;
+    int npix;
+    int slice;
+    double *models;
+    float *chi2;
+    int *iters;
+    int *accepted;
+    float *fit;
+    int *counters;
+    unsigned long long *fallbacks;
+};
+
+__device__ __forceinline__ void fetch_chunk(int *counters, int xcd, int npix, int slice,
+                                            int &sl_off, int &lo_out, int &hi_out)
+{
+    lo_out = -1;
+    hi_out = -1;
+    while (sl_off < PM_NSLICE) {
+        int s = (xcd + sl_off) & (PM_NSLICE - 1);
+        int lo = s * slice;
+        int hi = min(npix, lo + slice);
+        if (lo < hi) {
+            int v = atomicAdd(&counters[s * PM_CSTRIDE], PM_CHUNK);
+            if (lo + v < hi) {
+                lo_out = lo + v;
+                hi_out = min(hi, lo + v + PM_CHUNK);
+                return;
+            }
+        }
+        sl_off++;
+    }
+}
+
+__global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
+{
+    int i = threadIdx.x;
+    if (i < PM_NSLICE) counters[i * PM_CSTRIDE] = 0;
+    if (i == 0) *fallbacks = 0ull;
+}
+
+// ------------------------------------------------------------ inversion kernel
+
+// One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
+// loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
+template <typename T, int W, int NLPL, int NT>
+__global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
+{
+    using LD = Lds<T, W, NLPL, NT>;
+    constexpr int NP = LD::NP;
+    __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
+    const PmConst *__restrict__ c = a.c;
+    const int L = c->L;
+    const int grp = threadIdx.x / W, t = threadIdx.x % W;
+    char *gb = smem + grp * LD::GROUP_BYTES;
+    T *pad = (T *)gb;
+    float *sobs = (float *)(gb + LD::U_BYTES);
+    double *ab = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES);
+    for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+    wave_sync();
+
+    const int xcd = blockIdx.x % PM_NSLICE;
+    int sl_off = 0, cur = 0, end = 0;
+
+    double model[11], trial[11];
+    double flambda = 0, ochisqr = 0, chisqr0 = 0, factor = 1.0;
+    int it = 0, nacc = 0, nfree = 1, pix = -1;
+    bool iterating = false;
+    float obs[4][NLPL];
+    T S[4][NLPL];
+    Samp<T> sp[NLPL];
+    Geom<T> g;
+
+    while (true) {
+        bool first = false;
+        if (!iterating) {
+            if (cur >= end) {
+                int lo = -1, hi = -1;
+                if (t == 0) fetch_chunk(a.counters, xcd, a.npix, a.slice, sl_off, lo, hi);
+                lo = __shfl(lo, 0, W);
+                hi = __shfl(hi, 0, W);
+                sl_off = __shfl(sl_off, 0, W);
+                if (lo < 0) break;
+                cur = lo;
+                end = hi;
+            }
+            pix = cur++;
+            const float *src = a.stokes + (size_t)pix * 4 * L;
+            int ng = 0;
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) {
+                int lam = t + k * W;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    float v = lam < L ? src[s * L + lam] : 0.0f;
+                    obs[s][k] = v;
+                    if (lam < L) {
+                        sobs[s * L + lam] = v;
+                        ng += (v <= -1.0f);                 // milosUtils.c:484-492
+                    }
+                }
+            }
+            wave_sync();
+            nfree = 4 * L - NT - group_sum_int<W>(ng);        // :507-508
+#pragma unroll
+            for (int i = 0; i < 11; i++) model[i] = c->guess[i];
+            classical(c, sobs, model);
+            wave_sync();
+            if (nfree == 0) {                                 // :509-512, milos.c:997
+                if (t == 0) {
+                    for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = model[i];
+                    a.chi2[pix] = 0.0f;
+                    a.iters[pix] = pix;
+                    if (a.accepted) a.accepted[pix] = 0;
+                }
+                if (a.fit) {
+                    for (int k = 0; k < NLPL; k++) {
+                        int lam = t + k * W;
+                        if (lam < L)
+                            for (int s = 0; s < 4; s++)
+                                a.fit[(size_t)pix * 4 * L + s * L + lam] = 0.0f;
+                    }
+                }
+                continue;
+            }
+            flambda = c->ilambda;
+            it = 0;
+            nacc = 0;
+            factor = 1.0;
+#pragma unroll
+            for (int i = 0; i < 11; i++) trial[i] = model[i];
+            first = true;
+        } else {
+            // covar = alpha with the diagonal scaled by (1 + lambda), :559-563
+            double Ap[NP], bb[NT], delta[NT];
+#pragma unroll
+            for (int i = 0; i < NP; i++) Ap[i] = ab[i];
+#pragma unroll
+            for (int i = 0; i < NT; i++) {
+                Ap[i * (i + 1) / 2 + i] = ab[i * (i + 1) / 2 + i] * (1.0 + flambda);
+                bb[i] = ab[NP + i];
+            }
+            bool ok = ldl_solve<NT>(Ap, bb, delta);
+            if (!ok) {
+                double *EA = (double *)pad, *EV = EA + NT * NT;
+                for (int e = t; e < NT * NT; e += W) {
+                    int i = e / NT, j = e % NT;
+                    int ii = i > j ? i : j, jj = i > j ? j : i;
+                    EA[e] = Ap[ii * (ii + 1) / 2 + jj];
+                }
+                wave_sync();
+                jacobi_pinv<W, NT>(EA, EV, t, bb, delta);
+                for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+                wave_sync();
+                if (t == 0) atomicAdd(a.fallbacks, 1ull);
+            }
+            aplica_delta(c, NT, model, delta, trial);         // :566
+            check_model(trial);                               // :568
+        }
+
+        // trial synthesis and chi2, :569-571 (fchisqr, lib.c:126-152)
+        synth_slots<T, W, NLPL>(c, trial, pad, t, L, g, sp, S);
+        double part = 0;
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            int lam = t + k * W;
+            if (lam < L) {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
+                    double d = (double)S[s][k] - (double)obs[s][k];
+                    part += (((d * d) * c->weight[s]) / sg);
+                }
+            }
+        }
+        double c2 = group_sum<W>(part) / (double)nfree;
+
+        bool do_rf, finished = false;
+        if (first) {
+            ochisqr = c2;                                      // :546-547
+            chisqr0 = c2;
+            do_rf = true;
+            iterating = true;
+        } else {
+            bool clanda = (fabs((ochisqr - c2) * 100 / c2) < c->toplim) || (c2 < 0.0001);
+            bool acc = (c2 - ochisqr < 0.);
+            if (acc) {                                         // :581-606
+                flambda = flambda / (5.0 * factor);
+#pragma unroll
+                for (int i = 0; i < 11; i++) model[i] = trial[i];
+                ochisqr = c2;
+                nacc++;
+            } else {                                           // :607-615
+                flambda = flambda * 10.0 * factor;
+            }
+            if ((flambda > 1e+7) || (flambda < 1e-25)) clanda = true;
+            it++;
+            if (c->logclambda)                                 // :621
+                factor = (double)((float)log10((double)(float)c2) /
+                                  (float)log10((double)(float)chisqr0));
+            finished = !(it < c->miter && !clanda);
+            // the reference also recomputes J^T J after the final accepted
+            // trip; nothing reads it, so it is skipped
+            do_rf = acc && !finished;
+        }
+
+        if (do_rf) {
+            T D[NLPL][4][NT];
+            rf_slots<T, W, NLPL, NT>(c, g, sp, pad, t, L, D);
+            // covarm2 (lib.c:71-116): per-lane partial sums, then the group sum
+            double ap[NP], bp[NT];
+#pragma unroll
+            for (int i = 0; i < NP; i++) ap[i] = 0;
+#pragma unroll
+            for (int i = 0; i < NT; i++) bp[i] = 0;
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) {
+                int lam = t + k * W;
+                if (lam >= L) continue;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
+                    double w = c->weight[s];
+                    double wsig = w / sg;
+                    double opa = w * ((double)S[s][k] - (double)obs[s][k]);
+                    double d[NT];
+#pragma unroll
+                    for (int p = 0; p < NT; p++)
+                        d[p] = row_is_fixed(c, NT, p) ? 0.0 : (double)D[k][s][p];
+#pragma unroll
+                    for (int i = 0; i < NT; i++) {
+#pragma unroll
+                        for (int h = 0; h <= i; h++) ap[i * (i + 1) / 2 + h] += (d[i] * d[h]) * wsig;
+                        bp[i] += (opa * d[i]) / sg;
+                    }
+                }
+            }
+#pragma unroll
+            for (int i = 0; i < NP; i++) ap[i] = group_sum<W>(ap[i]);
+#pragma unroll
+            for (int i = 0; i < NT; i++) bp[i] = group_sum<W>(bp[i]);
+            if (t == 0) {
+#pragma unroll
+                for (int i = 0; i < NP; i++) ab[i] = ap[i];
+#pragma unroll
+                for (int i = 0; i < NT; i++) ab[NP + i] = bp[i];
+            }
+            wave_sync();
+        }
+
+        if (finished) {
+            if (t == 0) {
+                for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = model[i];
+                a.chi2[pix] = (float)ochisqr;                   // :625
+                a.iters[pix] = it;
+                if (a.accepted) a.accepted[pix] = nacc;
+            }
+            if (a.fit) {                                        // last trial, :569
+#pragma unroll
+                for (int k = 0; k < NLPL; k++) {
+                    int lam = t + k * W;
+                    if (lam < L)
+#pragma unroll
+                        for (int s = 0; s < 4; s++)
+                            a.fit[(size_t)pix * 4 * L + s * L + lam] = (float)S[s][k];
+                }
+            }
+            iterating = false;
+        }
+    }
+}
+
+// ------------------------------------------------------------ synthesis kernel
+
+template <typename T, int W, int NLPL, int NT>
+__global__ void __launch_bounds__(PM_BLOCK)
+pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, double *rf)
+{
+    using LD = Lds<T, W, NLPL, NT>;
+    __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
+    const int L = c->L;
+    const int grp = threadIdx.x / W, t = threadIdx.x % W;
+    T *pad = (T *)(smem + grp * LD::GROUP_BYTES);
+    for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+    wave_sync();
+    const int ngroups = gridDim.x * LD::GPB;
+    for (int pix = blockIdx.x * LD::GPB + grp; pix < n; pix += ngroups) {
+        double m[11];
+#pragma unroll
+        for (int i = 0; i < 11; i++) m[i] = models[(size_t)pix * 11 + i];
+        Geom<T> g;
+        Samp<T> sp[NLPL];
+        T S[4][NLPL];
+        synth_slots<T, W, NLPL>(c, m, pad, t, L, g, sp, S);
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            int lam = t + k * W;
+            if (lam < L)
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    spectra[(size_t)pix * 4 * L + s * L + lam] = (double)S[s][k];
+        }
+        if (rf) {
+            T D[NLPL][4][NT];
+            rf_slots<T, W, NLPL, NT>(c, g, sp, pad, t, L, D);
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) {
+                int lam = t + k * W;
+                if (lam < L)
+#pragma unroll
+                    for (int s = 0; s < 4; s++)
+#pragma unroll
+                        for (int p = 0; p < NT; p++)
+                            rf[(size_t)pix * 4 * NT * L + (size_t)s * NT * L + p * L + lam] =
+                                (double)D[k][s][p];
+            }
+        }
+    }
+}
+
+// ============================================================ host side
+
+struct pm_ctx {
+    PmConst h;
+    PmConst *d = nullptr;
+    int *counters = nullptr;
+    unsigned long long *fallbacks = nullptr;
+    int precision = PM_FP64;
+    int device = 0;
+    int ncu = 0;
+    hipStream_t last_stream = nullptr;
+};
+
+extern "C" const char *pm_strerror(int status)
+{
+    switch (status) {
+    case PM_OK: return "ok";
+    case PM_EINVAL: return "invalid argument";
+    case PM_ENOMEM: return "device allocation failed";
+    case PM_EDEVICE: return "HIP runtime error";
+    case PM_ENOTSUP: return "configuration not supported";
+    default: return "unknown error";
+    }
+}
+
+extern "C" int pm_grid(double lambda0, double initial_mA, double step_mA, double final_mA,
+                       double *lambda, int cap)
+{
+    if (!(step_mA != 0.0)) return PM_EINVAL;
+    int n = (int)(((final_mA - initial_mA) / step_mA) + 1);       // milos.c:173
+    if (n <= 0) return PM_EINVAL;
+    double step = step_mA / 1000.0;
+    double v = lambda0 + (initial_mA / 1000.0);                   // milos.c:194-197
+    for (int i = 0; i < n; i++) {
+        if (i > 0) v = v + step;
+        if (lambda && i < cap) lambda[i] = v;
+    }
+    return n;
+}
+
+extern "C" int pm_psf_gaussian(double fwhm_mA, const double *lambda, int n, double *G)
+{
+    if (!lambda || !G || n < 2 || !(fwhm_mA > 0)) return PM_EINVAL;
+    // fgauss_WL(FWHM, lambda[1]-lambda[0], lambda[0], lambda[n/2], n) (milos.c:269)
+    double sigma = fwhm_mA * 0.42466090 / 1000.0;                 // fgauss.c:92
+    double step = lambda[1] - lambda[0];
+    double centre = lambda[n / 2];
+    double sum = 0;
+    for (int i = 0; i < n; i++) {
+        double a = ((lambda[0] + i * step) - centre) / sigma;
+        double term = (a * a) / 2;
+        G[i] = term < 1e30 ? exp(-term) : 0.0;
+        sum += G[i];
+    }
+    for (int i = 0; i < n; i++) G[i] /= sum;
+    return PM_OK;
+}
+
+extern "C" int pm_nterms(const int fix[PM_NPAR], double guess_mac)
+{
+    int nt = 11;                                                  // milos.c:154-157
+    if (fix[10] == 0) nt--;
+    if (guess_mac == 0 && fix[9] == 0) nt--;
+    return nt;
+}
+
+// Zeeman pattern of the line: Lande factors, sublevels, strengths and shifts
+// per Delta-M group, normalised per group (create_cuantic.c:60-234; M values
+// built from (int)J as there).
+static int zeeman_setup(PmConst &c, double sl, double ll, double jl, double su, double lu,
+                        double ju)
+{
+    double g1 = jl != 0 ? (3.0 * jl * (jl + 1) + sl * (sl + 1) - ll * (ll + 1)) / (2 * jl * (jl + 1)) : 0.0;
+    double g2 = ju != 0 ? (3.0 * ju * (ju + 1.0) + su * (su + 1.0) - lu * (lu + 1.0)) / (2.0 * ju * (ju + 1.0)) : 0.0;
+    c.geff = (g1 + g2) / 2 + (g1 - g2) * (jl * (jl + 1.) - ju * (ju + 1.)) / 4;
+    int npi = (int)((2 * (jl < ju ? jl : ju)) + 1), nsig = (int)(jl + ju);
+    if (npi < 1 || nsig < 1 || npi > PM_MAXC || nsig > PM_MAXC) return PM_ENOTSUP;
+    int jj = (int)(ju - jl);
+    int ip = 0, ib = 0, ir = 0;
+    double wer_tmp[PM_MAXC];
+    for (int j = 0; j <= 2 * jl; j++) {
+        int ml = j - (int)jl;
+        for (int i = 0; i <= 2 * ju; i++) {
+            int mu = i - (int)ju;
+            int dm = mu - ml;
+            double m = ml;
+            double shift = g1 * ml - g2 * mu;
+            double s = 0;
+            if (dm == 0) {
+                if (jj == -1) s = jl * jl - m * m;
+                else if (jj == 0) s = m * m;
+                else if (jj == 1) s = (jl + 1) * (jl + 1) - m * m;
+                if (ip >= PM_MAXC) return PM_ENOTSUP;
+                c.wep[ip] = s; c.nup[ip] = shift; ip++;
+            } else if (dm == 1) {
+                if (jj == -1) s = (jl - m) * (jl - m - 1) / 4;
+                else if (jj == 0) s = (jl - m) * (jl + m + 1) / 4;
+                else if (jj == 1) s = (jl + m + 1) * (jl + m + 2) / 4;
+                if (ib >= PM_MAXC) return PM_ENOTSUP;
+                c.web[ib] = s; c.nub[ib] = shift; ib++;
+            } else if (dm == -1) {
+                if (jj == -1) s = (jl + m) * (jl + m - 1) / 4;
+                else if (jj == 0) s = (jl + m) * (jl - m + 1) / 4;
+                else if (jj == 1) s = (jl - m + 1) * (jl - m + 2) / 4;
+                if (ir >= PM_MAXC) return PM_ENOTSUP;
+                wer_tmp[ir] = s; ir++;
+            }
+        }
+    }
+    if (ip != npi || ib != nsig || ir != nsig) return PM_ENOTSUP;
+    double sp = 0, sb = 0, sr = 0;
+    for (int i = 0; i < npi; i++) sp = sp + c.wep[i];
+    for (int i = 0; i < npi; i++) c.wep[i] = c.wep[i] / sp;
+    for (int i = 0; i < nsig; i++) sb = sb + c.web[i];
+    for (int i = 0; i < nsig; i++) c.web[i] = c.web[i] / sb;
+    for (int i = 0; i < nsig; i++) sr = sr + wer_tmp[i];
+    for (int i = 0; i < nsig; i++) c.wer[i] = wer_tmp[i] / sr;
+    c.npi = npi;
+    c.nsig = nsig;
+    return PM_OK;
+}
+
+#define PM_HIP(call)                                                          \
+    do {                                                                      \
+        hipError_t e_ = (call);                                               \
+        if (e_ != hipSuccess) {                                               \
+            fprintf(stderr, "pmilos: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+            return PM_EDEVICE;                                                \
+        }                                                                     \
+    } while (0)
+
+extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
+{
+    if (!cfg || !out || !cfg->lambda) return PM_EINVAL;
+    *out = nullptr;
+    const int L = cfg->nlambda;
+    if (L < 2 || L > PM_MAX_LAMBDA) return PM_ENOTSUP;
+    if (cfg->miter < 1) return PM_EINVAL;
+    if (cfg->precision != PM_FP64 && cfg->precision != PM_MIXED) return PM_EINVAL;
+    pm_ctx *ctx = new pm_ctx();
+    PmConst &c = ctx->h;
+    memset(&c, 0, sizeof(c));
+    int rc = zeeman_setup(c, cfg->sl, cfg->ll, cfg->jl, cfg->su, cfg->lu, cfg->ju);
+    if (rc != PM_OK) { delete ctx; return rc; }
+    c.L = L;
+    c.NT = pm_nterms(cfg->fix, cfg->guess[9]);
+    // macroturbulence and stray light are not in these kernels yet
+    if (c.NT != 9 || cfg->guess[9] > 0.0001) { delete ctx; return PM_ENOTSUP; }
+    c.use_psf = cfg->psf ? 1 : 0;
+    c.miter = cfg->miter;
+    c.logclambda = cfg->logclambda;
+    for (int i = 0; i < 11; i++) { c.fix[i] = cfg->fix[i]; c.guess[i] = cfg->guess[i]; }
+    c.lambda0 = cfg->lambda0;
+    c.noise = cfg->noise;
+    c.ilambda = cfg->ilambda;
+    c.toplim = cfg->toplim;
+    c.mu = cfg->mu;
+    for (int i = 0; i < 4; i++) c.weight[i] = cfg->weight[i];
+    for (int i = 0; i < L; i++) {
+        c.lam[i] = cfg->lambda[i];
+        c.G[i] = cfg->psf ? cfg->psf[i] : 0.0;
+    }
+    ctx->precision = cfg->precision;
+    if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return PM_EDEVICE; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, ctx->device) != hipSuccess) { delete ctx; return PM_EDEVICE; }
+    ctx->ncu = prop.multiProcessorCount;
+    if (hipMalloc(&ctx->d, sizeof(PmConst)) != hipSuccess ||
+        hipMalloc(&ctx->counters, sizeof(int) * PM_NSLICE * PM_CSTRIDE) != hipSuccess ||
+        hipMalloc(&ctx->fallbacks, sizeof(unsigned long long)) != hipSuccess) {
+        pm_ctx_destroy(ctx);
+        return PM_ENOMEM;
+    }
+    if (hipMemcpy(ctx->d, &c, sizeof(PmConst), hipMemcpyHostToDevice) != hipSuccess) {
+        pm_ctx_destroy(ctx);
+        return PM_EDEVICE;
+    }
+    *out = ctx;
+    return PM_OK;
+}
+
+extern "C" void pm_ctx_destroy(pm_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->d) (void)hipFree(ctx->d);
+    if (ctx->counters) (void)hipFree(ctx->counters);
+    if (ctx->fallbacks) (void)hipFree(ctx->fallbacks);
+    delete ctx;
+}
+
+extern "C" int pm_ctx_nterms(const pm_ctx *ctx) { return ctx ? ctx->h.NT : PM_EINVAL; }
+
+// ------------------------------------------------------------ dispatch
+
+template <typename T, int W, int NLPL, int NT>
+static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
+{
+    using LD = Lds<T, W, NLPL, NT>;
+    auto kern = pm_invert_kernel<T, W, NLPL, NT>;
+    int per_cu = 0;
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PM_BLOCK, 0));
+    if (per_cu < 1) per_cu = 1;
+    long long need = ((long long)a.npix + LD::GPB - 1) / LD::GPB;
+    long long grid = (long long)per_cu * ctx->ncu;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, a.counters, a.fallbacks);
+    PM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(PM_BLOCK), 0, st, a);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+template <typename T, int W, int NLPL, int NT>
+static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double *rf,
+                        hipStream_t st)
+{
+    using LD = Lds<T, W, NLPL, NT>;
+    long long need = ((long long)n + LD::GPB - 1) / LD::GPB;
+    long long grid = (long long)ctx->ncu * 8;
+    if (grid > need) grid = need;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL((pm_synth_kernel<T, W, NLPL, NT>), dim3((unsigned)grid), dim3(PM_BLOCK),
+                       0, st, (const PmConst *)ctx->d, m, n, sp, rf);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
+
+// Lanes per pixel for L wavelengths: the smallest power of two >= L up to a
+// full wavefront, then two samples per lane.
+#define PM_DISPATCH_L(L, FN, ...)                                             \
+    ((L) <= 8 ? FN<double, 8, 1, 9>(__VA_ARGS__) :                            \
+     (L) <= 16 ? FN<double, 16, 1, 9>(__VA_ARGS__) :                          \
+     (L) <= 32 ? FN<double, 32, 1, 9>(__VA_ARGS__) :                          \
+     (L) <= 64 ? FN<double, 64, 1, 9>(__VA_ARGS__) :                          \
+                 FN<double, 64, 2, 9>(__VA_ARGS__))
+
+extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectra,
+                               double *d_rf, void *stream)
+{
+    if (!ctx || n < 0 || (n > 0 && (!d_models || !d_spectra))) return PM_EINVAL;
+    if (n == 0) return PM_OK;
+    hipStream_t st = (hipStream_t)stream;
+    ctx->last_stream = st;
+    const int L = ctx->h.L;
+    return PM_DISPATCH_L(L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
+}
+
+extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
+                                float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
+                                void *stream)
+{
+    if (!ctx || npix < 0) return PM_EINVAL;
+    if (npix == 0) return PM_OK;
+    if (!d_stokes || !d_models || !d_chi2 || !d_iters) return PM_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    ctx->last_stream = st;
+    InvArgs a;
+    a.c = ctx->d;
+    a.stokes = d_stokes;
+    a.npix = npix;
+    a.slice = (npix + PM_NSLICE - 1) / PM_NSLICE;
+    a.models = d_models;
+    a.chi2 = d_chi2;
+    a.iters = d_iters;
+    a.accepted = d_accepted;
+    a.fit = d_fit;
+    a.counters = ctx->counters;
+    a.fallbacks = ctx->fallbacks;
+    const int L = ctx->h.L;
+    return PM_DISPATCH_L(L, launch_invert, ctx, a, st);
+}
+
+extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
+{
+    if (!ctx || !count) return PM_EINVAL;
+    unsigned long long v = 0;
+    PM_HIP(hipStreamSynchronize(ctx->last_stream));
+    PM_HIP(hipMemcpy(&v, ctx->fallbacks, sizeof(v), hipMemcpyDeviceToHost));
+    *count = (long long)v;
+    return PM_OK;
+}
+
+// ------------------------------------------------------------ host-pointer wrappers
+
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    int alloc(size_t n) { return hipMalloc(&p, n ? n : 1) == hipSuccess ? PM_OK : PM_ENOMEM; }
+};
+}  // namespace
+
+extern "C" int pm_synth(pm_ctx *ctx, const double *models, int n, double *spectra, double *rf)
+{
+    if (!ctx || n < 0 || (n > 0 && (!models || !spectra))) return PM_EINVAL;
+    if (n == 0) return PM_OK;
+    const size_t L = ctx->h.L, NT = ctx->h.NT;
+    DevBuf dm, ds, dr;
+    if (dm.alloc(sizeof(double) * 11 * n) || ds.alloc(sizeof(double) * 4 * L * n) ||
+        (rf && dr.alloc(sizeof(double) * 4 * NT * L * n)))
+        return PM_ENOMEM;
+    PM_HIP(hipMemcpy(dm.p, models, sizeof(double) * 11 * n, hipMemcpyHostToDevice));
+    int rc = pm_synth_device(ctx, (double *)dm.p, n, (double *)ds.p, rf ? (double *)dr.p : nullptr,
+                             nullptr);
+    if (rc) return rc;
+    PM_HIP(hipDeviceSynchronize());
+    PM_HIP(hipMemcpy(spectra, ds.p, sizeof(double) * 4 * L * n, hipMemcpyDeviceToHost));
+    if (rf) PM_HIP(hipMemcpy(rf, dr.p, sizeof(double) * 4 * NT * L * n, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+extern "C" int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models, float *chi2,
+                         int *iters, int *accepted, float *fit)
+{
+    if (!ctx || npix < 0) return PM_EINVAL;
+    if (npix == 0) return PM_OK;
+    if (!stokes || !models || !chi2 || !iters) return PM_EINVAL;
+    const size_t L = ctx->h.L, n = npix;
+    DevBuf ds, dm, dc, di, da, df;
+    if (ds.alloc(sizeof(float) * 4 * L * n) || dm.alloc(sizeof(double) * 11 * n) ||
+        dc.alloc(sizeof(float) * n) || di.alloc(sizeof(int) * n) ||
+        (accepted && da.alloc(sizeof(int) * n)) || (fit && df.alloc(sizeof(float) * 4 * L * n)))
+        return PM_ENOMEM;
+    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
+    int rc = pm_invert_device(ctx, (const float *)ds.p, npix, (double *)dm.p, (float *)dc.p,
+                              (int *)di.p, accepted ? (int *)da.p : nullptr,
+                              fit ? (float *)df.p : nullptr, nullptr);
+    if (rc) return rc;
+    PM_HIP(hipDeviceSynchronize());
+    PM_HIP(hipMemcpy(models, dm.p, sizeof(double) * 11 * n, hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(chi2, dc.p, sizeof(float) * n, hipMemcpyDeviceToHost));
+    PM_HIP(hipMemcpy(iters, di.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (accepted) PM_HIP(hipMemcpy(accepted, da.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (fit) PM_HIP(hipMemcpy(fit, df.p, sizeof(float) * 4 * L * n, hipMemcpyDeviceToHost));
+    return PM_OK;
+}

@@ -1,0 +1,166 @@
+"""Python binding of libpmilos_hip.so (the C ABI of include/pmilos.h).
+
+Host-side mirror of the reference's per-pixel interface: a Context holds what
+milos.c sets up once per run (line, Zeeman pattern, wavelength grid, PSF,
+control settings); Context.invert replaces the FITS pixel loop around lm_mils
+(milos.c:958-1009) and Context.synth the synthesis mode (milos.c:601-602).
+
+The HIP library is required: importing works without it, but every call
+raises if libpmilos_hip.so is missing (there is no CPU fallback).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpmilos_hip.so")
+NPAR = 11
+PM_FP64, PM_MIXED = 0, 1
+PARAM_NAMES = ["eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "mac", "alfa"]
+
+_lib = None
+
+
+class PmError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    _fields_ = [("nlambda", C.c_int), ("lam", C.POINTER(C.c_double)),
+                ("lambda0", C.c_double),
+                ("sl", C.c_double), ("ll", C.c_double), ("jl", C.c_double),
+                ("su", C.c_double), ("lu", C.c_double), ("ju", C.c_double),
+                ("psf", C.POINTER(C.c_double)),
+                ("fix", C.c_int * NPAR), ("weight", C.c_double * 4),
+                ("noise", C.c_double), ("ilambda", C.c_double), ("toplim", C.c_double),
+                ("miter", C.c_int), ("mu", C.c_double), ("logclambda", C.c_int),
+                ("guess", C.c_double * NPAR), ("precision", C.c_int)]
+
+
+def lib():
+    """The loaded HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PmError("libpmilos_hip.so not built (run p-milos_amd/pmilos/build.py)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.pm_strerror.restype = C.c_char_p
+        L.pm_strerror.argtypes = [C.c_int]
+        L.pm_grid.argtypes = [C.c_double] * 4 + [vp, C.c_int]
+        L.pm_psf_gaussian.argtypes = [C.c_double, vp, C.c_int, vp]
+        L.pm_nterms.argtypes = [vp, C.c_double]
+        L.pm_ctx_create.argtypes = [C.POINTER(Config), C.POINTER(vp)]
+        L.pm_ctx_destroy.argtypes = [vp]
+        L.pm_ctx_destroy.restype = None
+        L.pm_ctx_nterms.argtypes = [vp]
+        L.pm_synth.argtypes = [vp, vp, C.c_int, vp, vp]
+        L.pm_synth_device.argtypes = [vp, vp, C.c_int, vp, vp, vp]
+        L.pm_invert.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
+        L.pm_invert_device.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.pm_last_fallback_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc < 0:
+        raise PmError("pmilos: %s (%d)" % (lib().pm_strerror(rc).decode(), rc))
+    return rc
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def grid(lambda0, initial_mA, step_mA, final_mA):
+    n = _check(lib().pm_grid(lambda0, initial_mA, step_mA, final_mA, None, 0))
+    lam = np.zeros(n)
+    lib().pm_grid(lambda0, initial_mA, step_mA, final_mA, lam.ctypes.data, n)
+    return lam
+
+
+def psf_gaussian(fwhm_mA, lam):
+    lam = np.ascontiguousarray(lam, dtype=np.float64)
+    G = np.zeros(len(lam))
+    _check(lib().pm_psf_gaussian(fwhm_mA, lam.ctypes.data, len(lam), G.ctypes.data))
+    return G
+
+
+class Context:
+    """Per-run state on the device (pm_ctx)."""
+
+    def __init__(self, lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
+                 noise=1e-6, ilambda=float(np.float32(0.1)), toplim=1e-12, miter=50,
+                 mu=1.0, logclambda=0, guess=None, precision=PM_FP64):
+        self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._psf = None if psf is None else np.ascontiguousarray(psf, dtype=np.float64)
+        cfg = Config()
+        cfg.nlambda = len(self._lam)
+        cfg.lam = self._lam.ctypes.data_as(C.POINTER(C.c_double))
+        cfg.lambda0 = lambda0
+        cfg.sl, cfg.ll, cfg.jl, cfg.su, cfg.lu, cfg.ju = [float(q) for q in quantum]
+        if self._psf is not None:
+            cfg.psf = self._psf.ctypes.data_as(C.POINTER(C.c_double))
+        fix = [1] * 9 + [0, 0] if fix is None else fix
+        for i in range(NPAR):
+            cfg.fix[i] = int(fix[i])
+        for i in range(4):
+            cfg.weight[i] = weight[i]
+        cfg.noise, cfg.ilambda, cfg.toplim = noise, ilambda, toplim
+        cfg.miter, cfg.mu, cfg.logclambda = miter, mu, logclambda
+        guess = np.zeros(NPAR) if guess is None else guess
+        for i in range(NPAR):
+            cfg.guess[i] = float(guess[i])
+        cfg.precision = precision
+        h = C.c_void_p()
+        _check(lib().pm_ctx_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.L = len(self._lam)
+        self.NT = lib().pm_ctx_nterms(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synth(self, models, rf=False):
+        m = np.ascontiguousarray(np.atleast_2d(models), dtype=np.float64)
+        n = m.shape[0]
+        spec = np.zeros((n, 4, self.L))
+        d = np.zeros((n, 4, self.NT, self.L)) if rf else None
+        _check(lib().pm_synth(self.h, m.ctypes.data, n, spec.ctypes.data, _ptr(d)))
+        return (spec, d) if rf else spec
+
+    def invert(self, stokes, want_fit=False):
+        st = np.ascontiguousarray(stokes, dtype=np.float32)
+        n = st.shape[0]
+        models = np.zeros((n, NPAR))
+        chi2 = np.zeros(n, dtype=np.float32)
+        iters = np.zeros(n, dtype=np.int32)
+        acc = np.zeros(n, dtype=np.int32)
+        fit = np.zeros((n, 4, self.L), dtype=np.float32) if want_fit else None
+        _check(lib().pm_invert(self.h, st.ctypes.data, n, models.ctypes.data,
+                               chi2.ctypes.data, iters.ctypes.data, acc.ctypes.data, _ptr(fit)))
+        return dict(model=models, chi2=chi2, iters=iters, accepted=acc, fit=fit)
+
+    def invert_ptrs(self, d_stokes, npix, d_models, d_chi2, d_iters, d_accepted=0, d_fit=0,
+                    stream=0):
+        _check(lib().pm_invert_device(self.h, d_stokes, npix, d_models, d_chi2, d_iters,
+                                      d_accepted or None, d_fit or None, stream or None))
+
+    def synth_ptrs(self, d_models, n, d_spectra, d_rf=0, stream=0):
+        _check(lib().pm_synth_device(self.h, d_models, n, d_spectra, d_rf or None,
+                                     stream or None))
+
+    def fallback_count(self):
+        v = C.c_longlong(0)
+        _check(lib().pm_last_fallback_count(self.h, C.byref(v)))
+        return v.value

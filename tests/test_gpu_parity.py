@@ -1,0 +1,155 @@
+"""GPU parity: the HIP library (through its C ABI) against the CPU oracle on
+the same seeded inputs.  FP64 synthesis and response functions must agree
+within 1e-10 relative; inversions per parameter on >= 99.5% of pixels."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pmilos
+from fixtures import (FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models,
+                      oracle_cube)
+
+pytestmark = pytest.mark.gpu
+
+# per-parameter tolerance of the FP64 inversion against the FP64 oracle
+# (absolute, in the parameter's unit): eta0, B [G], vlos [km/s], dopp [A],
+# a, gamma [deg], phi [deg], S0, S1
+TOL64 = np.array([1e-3, 1e-2, 1e-6, 1e-8, 1e-6, 1e-4, 1e-4, 1e-6, 1e-6])
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _grids():
+    return {
+        6: (FE6173, -172.5, 69, 172.5),
+        14: (FE6173, -260, 40, 260),
+        30: (FE6173, -290, 20, 290),
+        49: (FE6173, -600, 25, 600),
+        112: (FE6302, -1193.25, 21.5, 1193.25),
+    }
+
+
+def _pair(L, psf=True, **kw):
+    lam = O.grid(*_grids()[L])
+    l0 = _grids()[L][0]
+    G = O.gauss_psf(60.0, lam) if psf else None
+    s = O.make_setup(l0, TRIPLET, lam, psf=G, **{k: v for k, v in kw.items() if k != "guess"})
+    ctx = pmilos.Context(l0, TRIPLET, lam, psf=G, guess=kw.get("guess", GUESS),
+                         **{k: v for k, v in kw.items() if k in ("fix", "noise", "miter", "logclambda", "ilambda")})
+    return s, ctx
+
+
+def _relerr_rows(a, b):
+    """max |a-b| / max|b| over the last axis (each Stokes / parameter row)."""
+    scale = np.abs(b).max(axis=-1, keepdims=True)
+    scale = np.where(scale == 0, 1.0, scale)
+    return (np.abs(a - b) / scale).max()
+
+
+@pytest.mark.parametrize("L", [6, 14, 30, 49, 112])
+@pytest.mark.parametrize("psf", [True, False])
+def test_synthesis_and_rf_fp64(L, psf):
+    say("case synth L=%d psf=%s" % (L, psf))
+    s, ctx = _pair(L, psf=psf)
+    models = draw_models(48, seed=1000 + L)
+    models[0] = [14., 1200., 0.5, 0.07, 0.05, 130., 25., 0.2, 0.75, 0, 1.]
+    models[1, 1] = 0.0      # B = 0
+    models[2, 5] = 0.0      # gamma = 0
+    spec_g, rf_g = ctx.synth(models, rf=True)
+    for i, m in enumerate(models):
+        sp, rf = O.synth(s, m, rf=True)
+        e1 = _relerr_rows(spec_g[i], sp)
+        e2 = _relerr_rows(rf_g[i], rf)
+        assert e1 < 1e-10, (i, e1)
+        assert e2 < 1e-10, (i, e2)
+
+
+def _compare_inversions(g, o, label, tol=TOL64, frac=0.995):
+    dm = np.abs(g["model"][:, :9] - o["model"][:, :9])
+    ok = dm <= tol
+    say("%s: per-parameter fraction within tolerance %s" % (label, np.round(ok.mean(axis=0), 4)))
+    same_it = (g["iters"] == o["iters"]).mean()
+    say("%s: iterations identical on %.4f of pixels; mean iters gpu %.2f oracle %.2f"
+        % (label, same_it, g["iters"].mean(), o["iters"].mean()))
+    assert np.all(ok.mean(axis=0) >= frac), ok.mean(axis=0)
+    both = np.all(ok, axis=1)
+    np.testing.assert_allclose(g["chi2"][both], o["chi2"][both], rtol=1e-4)
+
+
+def test_invert_fp64_config1_sample():
+    """Config-1 recipe (seed 1, sigma 1e-3 Ic, PSF 60 mA, 30 lambda, 9 free,
+    <= 50 iterations) on 1024 pixels, both from the same float32 cube."""
+    say("case invert L=30 npix=1024 seed=1")
+    s, ctx = _pair(30)
+    truths = draw_models(1024, seed=1)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=11)
+    o = O.invert(s, cube, GUESS)
+    g = ctx.invert(cube, want_fit=True)
+    _compare_inversions(g, o, "L30")
+    assert ctx.fallback_count() >= 0
+    # fits are the last trial synthesis of each pixel
+    o_fit = O.invert(s, cube[:16], GUESS, want_fit=True)["fit"]
+    same = g["iters"][:16] == o["iters"][:16]
+    np.testing.assert_allclose(g["fit"][:16][same], o_fit[same], rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.parametrize("L", [6, 112])
+def test_invert_fp64_other_samplings(L):
+    say("case invert L=%d npix=256" % L)
+    s, ctx = _pair(L)
+    truths = draw_models(256, seed=40 + L)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=7)
+    o = O.invert(s, cube, GUESS)
+    g = ctx.invert(cube)
+    _compare_inversions(g, o, "L%d" % L)
+
+
+def test_invert_edge_cases():
+    """Ghost points, a pixel with nfree == 0 behaviour inputs, NaN spectra,
+    a fixed parameter and logclambda."""
+    say("case invert edge cases")
+    s, ctx = _pair(30, fix=[1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 0], logclambda=0)
+    truths = draw_models(64, seed=5)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=6)
+    cube[3, 1, 4] = -5.0                 # ghost point
+    cube[4, 0, :3] = -2.0                # leading ghost wavelengths in I
+    cube[5, 2, :] = -3.0                 # ghost U
+    cube[6] = np.nan                     # NaN pixel
+    o = O.invert(s, cube, GUESS)
+    g = ctx.invert(cube)
+    fin = np.isfinite(o["model"][:, 0])
+    assert np.array_equal(fin, np.isfinite(g["model"][:, 0]))
+    sub = lambda r: {k: v[fin] for k, v in r.items() if v is not None}
+    _compare_inversions(sub(g), sub(o), "edge", frac=0.98)
+    assert np.all(g["model"][:, 4] == GUESS[4])          # damping fixed
+
+
+def test_invert_logclambda():
+    say("case invert logclambda=1")
+    s, ctx = _pair(30, logclambda=1)
+    truths = draw_models(256, seed=9)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=10)
+    o = O.invert(s, cube, GUESS)
+    g = ctx.invert(cube)
+    # glibc log10f vs the correctly rounded value can move the damping factor
+    # by 1-2 ulp of float, so a few trajectories may part near convergence
+    _compare_inversions(g, o, "logc", tol=TOL64 * 1e3, frac=0.97)
+
+
+def test_nfree_zero_pixel():
+    say("case nfree==0")
+    lam = O.grid(FE6173, -20, 20, 20)      # L = 3: 12 - 9 = 3 ghosts -> nfree 0
+    ctx = pmilos.Context(FE6173, TRIPLET, lam, guess=GUESS)
+    s = O.make_setup(FE6173, TRIPLET, lam)
+    cube = np.full((5, 4, 3), 0.5, dtype=np.float32)
+    cube[2, 1, :] = -2.0                  # 3 ghost samples in Q
+    g = ctx.invert(cube)
+    o = O.invert(s, cube, GUESS)
+    assert g["iters"][2] == 2 and o["iters"][2] == 2      # the pixel index
+    assert g["chi2"][2] == 0.0
+    np.testing.assert_allclose(g["model"][2], o["model"][2], rtol=1e-12)
