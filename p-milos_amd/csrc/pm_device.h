@@ -451,7 +451,8 @@ __device__ __forceinline__ void check_model(double *m)
 
 // ------------------------------------------------------------ solve
 
-// delta = A^+ beta for the damped normal matrix.  The reference forms the
+// delta = A^+ beta for the damped normal matrix (alpha with its diagonal
+// scaled by damp = 1 + lambda).  The reference forms the
 // eigen pseudo-inverse with an absolute cutoff |w| > 1e-12
 // (milosUtils.c:237-271).  For an exactly zero row (a fixed parameter) the
 // eigenvalue is 0 and that component is 0; on the remaining block, when an
@@ -460,89 +461,89 @@ __device__ __forceinline__ void check_model(double *m)
 // and the pseudo-inverse is the inverse.  Otherwise returns false and the
 // caller uses the eigen pseudo-inverse itself.
 template <int N>
-__device__ __forceinline__ bool ldl_solve(const double *Ap /* packed lower, N(N+1)/2 */,
-                                          const double *b, double *x)
+__device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower alpha, then beta */,
+                                          double damp, double *x)
 {
     constexpr int P = N * (N + 1) / 2;
-    double Lm[P];
-    double d[N];
+    // F holds L (strictly lower, unit diagonal implied) with D on the diagonal
+    double F[P];
     bool act[N];
 #pragma unroll
-    for (int i = 0; i < N; i++) act[i] = Ap[i * (i + 1) / 2 + i] != 0.0;
+    for (int i = 0; i < N; i++) {
+        const int ii = i * (i + 1) / 2;
+#pragma unroll
+        for (int j = 0; j < i; j++) F[ii + j] = ab[ii + j];
+        F[ii + i] = ab[ii + i] * damp;                       // milosUtils.c:559-563
+        act[i] = F[ii + i] != 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < N; j++) {
         const int jj = j * (j + 1) / 2;
         if (!act[j]) {
-            d[j] = 1.0;
+            F[jj + j] = 1.0;
 #pragma unroll
-            for (int k = 0; k < j; k++) Lm[jj + k] = 0.0;
-            Lm[jj + j] = 1.0;
+            for (int k = 0; k < j; k++) F[jj + k] = 0.0;
+#pragma unroll
+            for (int i = j + 1; i < N; i++) F[i * (i + 1) / 2 + j] = 0.0;
             continue;
         }
-        double dj = Ap[jj + j];
+        double dj = F[jj + j];
 #pragma unroll
-        for (int k = 0; k < j; k++) {
-            double l = Lm[jj + k];
-            dj -= l * l * d[k];
-        }
+        for (int k = 0; k < j; k++) dj -= F[jj + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
         if (!(dj > 0.0)) return false;
-        d[j] = dj;
-        Lm[jj + j] = 1.0;
+        F[jj + j] = dj;
 #pragma unroll
         for (int i = j + 1; i < N; i++) {
             const int ii = i * (i + 1) / 2;
-            double v = act[i] ? Ap[ii + j] : 0.0;
+            double v = F[ii + j];
 #pragma unroll
-            for (int k = 0; k < j; k++) v -= Lm[ii + k] * Lm[jj + k] * d[k];
-            // stored transposed-free: row i, column j
-            Lm[ii + j] = act[i] ? v / dj : 0.0;
+            for (int k = 0; k < j; k++) v -= F[ii + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
+            F[ii + j] = act[i] ? v / dj : 0.0;
         }
     }
-    // trace(A^-1) = sum_k (1/d_k) sum_i (L^-1)_{ki}^2 over the active block
-    double Mi[P];
-    double tr = 0.0;
+    // solve L D L^T x = beta
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int ii = i * (i + 1) / 2;
-        Mi[ii + i] = 1.0;
+        double v = act[i] ? ab[P + i] : 0.0;
 #pragma unroll
-        for (int j = 0; j < i; j++) {
-            double v = 0.0;
+        for (int k = 0; k < i; k++) v -= F[ii + k] * x[k];
+        x[i] = v;
+    }
 #pragma unroll
-            for (int k = j; k < i; k++) v -= Lm[ii + k] * Mi[k * (k + 1) / 2 + j];
-            Mi[ii + j] = v;
+    for (int i = 0; i < N; i++) x[i] /= F[i * (i + 1) / 2 + i];
+#pragma unroll
+    for (int i = N - 1; i >= 0; i--) {
+        double v = x[i];
+#pragma unroll
+        for (int k = i + 1; k < N; k++) v -= F[k * (k + 1) / 2 + i] * x[k];
+        x[i] = act[i] ? v : 0.0;
+    }
+    // L^-1 in place (column by column), then trace(A^-1) =
+    // sum_k (1/d_k) sum_{i<=k} (L^-1)_{ki}^2; 1/trace bounds the smallest
+    // eigenvalue from below
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+#pragma unroll
+        for (int i = j + 1; i < N; i++) {
+            const int ii = i * (i + 1) / 2;
+            double v = F[ii + j];
+#pragma unroll
+            for (int k = j + 1; k < i; k++) v += F[ii + k] * F[k * (k + 1) / 2 + j];
+            F[ii + j] = -v;
         }
     }
+    double tr = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) {
         if (!act[k]) continue;
-        double s = 0.0;
         const int kk = k * (k + 1) / 2;
+        double s = 1.0;
 #pragma unroll
-        for (int i = 0; i <= k; i++) s += Mi[kk + i] * Mi[kk + i];
-        tr += s / d[k];
+        for (int i = 0; i < k; i++) s += F[kk + i] * F[kk + i];
+        tr += s / F[kk + k];
     }
-    if (!(tr < 1e9)) return false;
-    // forward, diagonal, backward
-    double y[N];
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const int ii = i * (i + 1) / 2;
-        double v = act[i] ? b[i] : 0.0;
-#pragma unroll
-        for (int k = 0; k < i; k++) v -= Lm[ii + k] * y[k];
-        y[i] = v;
-    }
-#pragma unroll
-    for (int i = 0; i < N; i++) y[i] /= d[i];
-#pragma unroll
-    for (int i = N - 1; i >= 0; i--) {
-        double v = y[i];
-#pragma unroll
-        for (int k = i + 1; k < N; k++) v -= Lm[k * (k + 1) / 2 + i] * x[k];
-        x[i] = act[i] ? v : 0.0;
-    }
-    return true;
+    return tr < 1e9;
 }
 
 // Eigen pseudo-inverse by cyclic Jacobi rotations on a matrix kept in LDS,

@@ -297,21 +297,18 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             first = true;
         } else {
             // covar = alpha with the diagonal scaled by (1 + lambda), :559-563
-            double Ap[NP], bb[NT], delta[NT];
-#pragma unroll
-            for (int i = 0; i < NP; i++) Ap[i] = ab[i];
-#pragma unroll
-            for (int i = 0; i < NT; i++) {
-                Ap[i * (i + 1) / 2 + i] = ab[i * (i + 1) / 2 + i] * (1.0 + flambda);
-                bb[i] = ab[NP + i];
-            }
-            bool ok = ldl_solve<NT>(Ap, bb, delta);
+            double delta[NT];
+            bool ok = ldl_solve<NT>(ab, 1.0 + flambda, delta);
             if (!ok) {
+                double bb[NT];
+#pragma unroll
+                for (int i = 0; i < NT; i++) bb[i] = ab[NP + i];
                 double *EA = (double *)pad, *EV = EA + NT * NT;
                 for (int e = t; e < NT * NT; e += W) {
                     int i = e / NT, j = e % NT;
                     int ii = i > j ? i : j, jj = i > j ? j : i;
-                    EA[e] = Ap[ii * (ii + 1) / 2 + jj];
+                    double v = ab[ii * (ii + 1) / 2 + jj];
+                    EA[e] = (i == j) ? v * (1.0 + flambda) : v;
                 }
                 wave_sync();
                 jacobi_pinv<W, NT>(EA, EV, t, bb, delta);
@@ -372,43 +369,49 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
         if (do_rf) {
             T D[NLPL][4][NT];
             rf_slots<T, W, NLPL, NT>(c, g, sp, pad, t, L, D);
-            // covarm2 (lib.c:71-116): per-lane partial sums, then the group sum
-            double ap[NP], bp[NT];
-#pragma unroll
-            for (int i = 0; i < NP; i++) ap[i] = 0;
-#pragma unroll
-            for (int i = 0; i < NT; i++) bp[i] = 0;
+            // covarm2 (lib.c:71-116): each alpha/beta entry is a per-lane
+            // partial sum followed by the group sum; lane 0 stores it in LDS
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
                 int lam = t + k * W;
-                if (lam >= L) continue;
+#pragma unroll
+                for (int p = 0; p < NT; p++) {
+                    bool z = lam >= L || row_is_fixed(c, NT, p);
+#pragma unroll
+                    for (int s = 0; s < 4; s++) if (z) D[k][s][p] = T(0);
+                }
+            }
+            double wsig[NLPL][4], rs[NLPL][4];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
                     double w = c->weight[s];
-                    double wsig = w / sg;
-                    double opa = w * ((double)S[s][k] - (double)obs[s][k]);
-                    double d[NT];
-#pragma unroll
-                    for (int p = 0; p < NT; p++)
-                        d[p] = row_is_fixed(c, NT, p) ? 0.0 : (double)D[k][s][p];
-#pragma unroll
-                    for (int i = 0; i < NT; i++) {
-#pragma unroll
-                        for (int h = 0; h <= i; h++) ap[i * (i + 1) / 2 + h] += (d[i] * d[h]) * wsig;
-                        bp[i] += (opa * d[i]) / sg;
-                    }
+                    wsig[k][s] = w / sg;
+                    rs[k][s] = (w * ((double)S[s][k] - (double)obs[s][k])) / sg;
                 }
             }
 #pragma unroll
-            for (int i = 0; i < NP; i++) ap[i] = group_sum<W>(ap[i]);
+            for (int i = 0; i < NT; i++) {
 #pragma unroll
-            for (int i = 0; i < NT; i++) bp[i] = group_sum<W>(bp[i]);
-            if (t == 0) {
+                for (int h = 0; h <= i; h++) {
+                    double v = 0;
 #pragma unroll
-                for (int i = 0; i < NP; i++) ab[i] = ap[i];
+                    for (int k = 0; k < NLPL; k++)
 #pragma unroll
-                for (int i = 0; i < NT; i++) ab[NP + i] = bp[i];
+                        for (int s = 0; s < 4; s++)
+                            v += ((double)D[k][s][i] * (double)D[k][s][h]) * wsig[k][s];
+                    v = group_sum<W>(v);
+                    if (t == 0) ab[i * (i + 1) / 2 + h] = v;
+                }
+                double b = 0;
+#pragma unroll
+                for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) b += rs[k][s] * (double)D[k][s][i];
+                b = group_sum<W>(b);
+                if (t == 0) ab[NP + i] = b;
             }
             wave_sync();
         }
