@@ -50,6 +50,30 @@ __device__ __forceinline__ double group_sum(double v)
     return v;
 }
 
+// Compensated FP32 group sum: a TwoSum butterfly carrying the rounding
+// error of every addition.  TwoSum is exact in either operand order, so all
+// lanes end with bit-identical (sum, error) pairs; returned as their double
+// sum.
+template <int W>
+__device__ __forceinline__ double group_sum_comp(float v)
+{
+    float sv = v, e = 0.0f;
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) {
+        float so = __shfl_xor(sv, o, W);
+        float eo = __shfl_xor(e, o, W);
+        float t = __fadd_rn(sv, so);
+        float bp = __fsub_rn(t, sv);
+        float err = __fadd_rn(__fsub_rn(sv, __fsub_rn(t, bp)), __fsub_rn(so, bp));
+        sv = t;
+        e = __fadd_rn(__fadd_rn(e, eo), err);
+    }
+    return (double)sv + (double)e;
+}
+
+template <typename T> struct AccT { typedef double type; };
+template <> struct AccT<float> { typedef float type; };
+
 template <int W>
 __device__ __forceinline__ int group_sum_int(int v)
 {
@@ -269,9 +293,11 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
                                          const Samp<T> &s, const UR<T> &r, T D[4][NT])
 {
     const T LD = g.LD, A = g.A, E0 = g.E0;
-    // per-group derivatives of the profiles w.r.t. B, vlos, dopp, a
-    // (funcionComponentFor, me_der.c:536-609; weights of the last component)
-    T dfi[3][4], dshi[3][4];
+    // per-group profile derivatives w.r.t. u of the last component
+    // (funcionComponentFor, me_der.c:536-609): dfi[q][j] = dH_u[q] * f[q][j],
+    // dshi[q][j] = dF_u[q] * f[q][j] for j = B, vlos, dopp (f = -nux, auxCte,
+    // -uu/LD); the damping row uses -2 dF_u and dH_u / 2 (me_der.c:581-600)
+    T dH_u[3], dF_u[3], fB[3], fD[3];
     const T auxCte = T(-c->lambda0) / (T(PM_VLIGHT) * LD);
 #pragma unroll
     for (int q = 0; q < 3; q++) {
@@ -286,68 +312,25 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
             w = T(c->wer[c->nsig - 1]);
             nux = -(g.parcial * T(c->nub[0]));
         }
-        T dH_u = ((T(4) * A * s.F[q]) - (T(2) * s.uu[q] * s.H[q])) * w;
-        T dF_u = (T(PM_RR) - A * s.H[q] - T(2) * s.uu[q] * s.F[q]) * w;
-        T uun = -s.uu[q] / LD;
-        dfi[q][0] = dH_u * (-nux);
-        dfi[q][1] = dH_u * auxCte;
-        dfi[q][2] = (dH_u * uun);
-        dfi[q][3] = (T(-2) * dF_u);
-        dshi[q][0] = (dF_u) * (-nux);
-        dshi[q][1] = dF_u * auxCte;
-        dshi[q][2] = (dF_u * uun);
-        dshi[q][3] = (dH_u / T(2));
+        dH_u[q] = ((T(4) * A * s.F[q]) - (T(2) * s.uu[q] * s.H[q])) * w;
+        dF_u[q] = (T(PM_RR) - A * s.H[q] - T(2) * s.uu[q] * s.F[q]) * w;
+        fB[q] = -nux;
+        fD[q] = -s.uu[q] / LD;
     }
-    // derivatives of eta / rho (me_der.c:93-199)
-    T dei[7], deq[7], deu[7], dev[7], drq[7], dru[7], drv[7];
-    T E0_2 = E0 / T(2);
-    T cosi_E0_2 = E0_2 * g.cosi;
-    dei[0] = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2)) / E0;
-    deq[0] = (r.parcial2 * g.sinis_cosa) / E0;
-    deu[0] = (r.parcial2 * g.sinis_sina) / E0;
-    dev[0] = ((s.fi[2] - s.fi[1]) * cosi_E0_2) / E0;
-    drq[0] = (r.parcial3 * g.sinis_cosa) / E0;
-    dru[0] = (r.parcial3 * g.sinis_sina) / E0;
-    drv[0] = ((s.shi[2] - s.shi[1]) * cosi_E0_2) / E0;
-    T sc = E0 * g.sinis_cosa / T(2);
-    T ss = E0 * g.sinis_sina / T(2);
-    T sindi_cosa = g.sindi * g.cosa, sindi_sina = g.sindi * g.sina;
-    T cosdi_E0_2 = (E0_2) * g.cosdi;
-    T cosi_2_E0 = E0 * g.cosi / T(2);
-    T cosis_2_E0_2 = g.cosis_2 * E0_2;
-    T sinis_E0_2 = g.sinis * E0_2;
-#pragma unroll
-    for (int j = 1; j < 5; j++) {
-        T dfisum = dfi[1][j - 1] + dfi[2][j - 1];
-        dei[j] = (dfi[0][j - 1] * sinis_E0_2 + dfisum * cosis_2_E0_2);
-        T aux = dfi[0][j - 1] - dfisum / T(2);
-        deq[j] = (aux) * sc;
-        deu[j] = (aux) * ss;
-        dev[j] = (dfi[2][j - 1] - dfi[1][j - 1]) * cosi_2_E0;
-        T auxr = dshi[0][j - 1] - (dshi[1][j - 1] + dshi[2][j - 1]) / T(2);
-        drq[j] = (auxr) * sc;
-        dru[j] = (auxr) * ss;
-        drv[j] = ((dshi[2][j - 1] - dshi[1][j - 1])) * cosi_2_E0;
-    }
-    T cosi_cosdi = g.cosi * g.cosdi * E0_2;
-    T sindi_E0_2 = g.sindi * E0_2;
-    dei[5] = s.fi[0] * sindi_E0_2 + (r.parcial1) * cosi_cosdi;
-    deq[5] = r.parcial2 * sindi_cosa;
-    deu[5] = r.parcial2 * sindi_sina;
-    dev[5] = (s.fi[2] - s.fi[1]) * cosdi_E0_2;
-    drq[5] = r.parcial3 * sindi_cosa;
-    dru[5] = r.parcial3 * sindi_sina;
-    drv[5] = (s.shi[2] - s.shi[1]) * cosdi_E0_2;
-    T sinis_cosda = g.sinis * g.cosda, sinis_sinda = g.sinis * g.sinda;
-    dei[6] = 0;
-    deq[6] = r.parcial2 * sinis_cosda;
-    deu[6] = r.parcial2 * sinis_sinda;
-    dev[6] = 0;
-    drq[6] = r.parcial3 * sinis_cosda;
-    dru[6] = r.parcial3 * sinis_sinda;
-    drv[6] = 0;
+    const T E0_2 = E0 / T(2);
+    const T cosi_E0_2 = E0_2 * g.cosi;
+    const T sc = E0 * g.sinis_cosa / T(2);                       // me_der.c:118-127
+    const T ss = E0 * g.sinis_sina / T(2);
+    const T sindi_cosa = g.sindi * g.cosa, sindi_sina = g.sindi * g.sina;
+    const T cosdi_E0_2 = (E0_2) * g.cosdi;
+    const T cosi_2_E0 = E0 * g.cosi / T(2);
+    const T cosis_2_E0_2 = g.cosis_2 * E0_2;
+    const T sinis_E0_2 = g.sinis * E0_2;
+    const T cosi_cosdi = g.cosi * g.cosdi * E0_2;
+    const T sindi_E0_2 = g.sindi * E0_2;
+    const T sinis_cosda = g.sinis * g.cosda, sinis_sinda = g.sinis * g.sinda;
 
-    // chain rule through the Unno-Rachkovsky solution (me_der.c:207-277)
+    // Unno-Rachkovsky terms shared by every parameter (me_der.c:207-220)
     const T ei = r.etai, eq = r.etaq, eu = r.etau, ev = r.etav;
     const T rq = r.rhoq, ru = r.rhou, rv = r.rhov;
     const T ei2 = r.etai_2, gp1 = r.gp1, gp2 = r.gp2, gp3 = r.gp3, dt = r.dt;
@@ -360,8 +343,58 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
     const T ext4 = aux2 * gp1;
 #pragma unroll
     for (int il = 0; il < 7; il++) {
-        T a_ei = dei[il], a_eq = deq[il], a_eu = deu[il], a_ev = dev[il];
-        T a_rq = drq[il], a_ru = dru[il], a_rv = drv[il];
+        // derivatives of eta_I,Q,U,V and rho_Q,U,V w.r.t. parameter il
+        // (me_der.c:93-199)
+        T a_ei, a_eq, a_eu, a_ev, a_rq, a_ru, a_rv;
+        if (il == 0) {
+            a_ei = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2)) / E0;
+            a_eq = (r.parcial2 * g.sinis_cosa) / E0;
+            a_eu = (r.parcial2 * g.sinis_sina) / E0;
+            a_ev = ((s.fi[2] - s.fi[1]) * cosi_E0_2) / E0;
+            a_rq = (r.parcial3 * g.sinis_cosa) / E0;
+            a_ru = (r.parcial3 * g.sinis_sina) / E0;
+            a_rv = ((s.shi[2] - s.shi[1]) * cosi_E0_2) / E0;
+        } else if (il <= 4) {
+            T fp, fb, fr, hp, hb, hr;
+            if (il == 4) {                                       // damping
+                fp = T(-2) * dF_u[0]; fb = T(-2) * dF_u[1]; fr = T(-2) * dF_u[2];
+                hp = dH_u[0] / T(2); hb = dH_u[1] / T(2); hr = dH_u[2] / T(2);
+            } else {
+                T f0, f1, f2;
+                if (il == 1) { f0 = fB[0]; f1 = fB[1]; f2 = fB[2]; }
+                else if (il == 2) { f0 = auxCte; f1 = auxCte; f2 = auxCte; }
+                else { f0 = fD[0]; f1 = fD[1]; f2 = fD[2]; }
+                fp = dH_u[0] * f0; fb = dH_u[1] * f1; fr = dH_u[2] * f2;
+                hp = dF_u[0] * f0; hb = dF_u[1] * f1; hr = dF_u[2] * f2;
+            }
+            T dfisum = fb + fr;
+            a_ei = (fp * sinis_E0_2 + dfisum * cosis_2_E0_2);
+            T aux = fp - dfisum / T(2);
+            a_eq = (aux) * sc;
+            a_eu = (aux) * ss;
+            a_ev = (fr - fb) * cosi_2_E0;
+            T auxr = hp - (hb + hr) / T(2);
+            a_rq = (auxr) * sc;
+            a_ru = (auxr) * ss;
+            a_rv = ((hr - hb)) * cosi_2_E0;
+        } else if (il == 5) {                                    // gamma
+            a_ei = s.fi[0] * sindi_E0_2 + (r.parcial1) * cosi_cosdi;
+            a_eq = r.parcial2 * sindi_cosa;
+            a_eu = r.parcial2 * sindi_sina;
+            a_ev = (s.fi[2] - s.fi[1]) * cosdi_E0_2;
+            a_rq = r.parcial3 * sindi_cosa;
+            a_ru = r.parcial3 * sindi_sina;
+            a_rv = (s.shi[2] - s.shi[1]) * cosdi_E0_2;
+        } else {                                                 // azimuth
+            a_ei = 0;
+            a_eq = r.parcial2 * sinis_cosda;
+            a_eu = r.parcial2 * sinis_sinda;
+            a_ev = 0;
+            a_rq = r.parcial3 * sinis_cosda;
+            a_ru = r.parcial3 * sinis_sinda;
+            a_rv = 0;
+        }
+        // chain rule through the Unno-Rachkovsky solution (me_der.c:223-277)
         T dgp1 = T(2) * (ei * a_ei - eq * a_eq - eu * a_eu - ev * a_ev + rq * a_rq +
                          ru * a_ru + rv * a_rv);
         T dgp2 = rq * a_eq + eq * a_rq + ru * a_eu + eu * a_ru + rv * a_ev + ev * a_rv;

@@ -47,11 +47,13 @@ struct Lds {
 // PSF, in place (convolution.c:54-78); when ic_row0 the I row uses the
 // continuum trick of convolution.c:90-113 with Ic = max(first, last) of the
 // row (mil_sinrf.c:390-395, me_der.c:478-483).  pad: 4 rows of PADW values,
-// zero outside [L/2, L/2 + L).  Accumulation in double (convolution.c:68).
+// zero outside [L/2, L/2 + L).  The FP64 form accumulates in double as
+// convolution.c:68 does; the mixed form in FP32 (kernel taps rounded once).
 template <typename T, int W, int NLPL>
 __device__ __forceinline__ void conv4(const PmConst *__restrict__ c, T *pad, int t, int L,
                                       T x[4][NLPL], bool ic_row0)
 {
+    typedef typename AccT<T>::type A;
     constexpr int PADW = 2 * W * NLPL;
     const int half = L / 2;
 #pragma unroll
@@ -79,16 +81,16 @@ __device__ __forceinline__ void conv4(const PmConst *__restrict__ c, T *pad, int
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         if (lam < L) {
-            double acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            A acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
             const T *p0 = pad + lam;
             for (int j = 0; j < L; j++) {
-                double gj = c->G[j];
-                acc0 += gj * (double)p0[j];
-                acc1 += gj * (double)p0[PADW + j];
-                acc2 += gj * (double)p0[2 * PADW + j];
-                acc3 += gj * (double)p0[3 * PADW + j];
+                A gj = A(c->G[j]);
+                acc0 += gj * (A)p0[j];
+                acc1 += gj * (A)p0[PADW + j];
+                acc2 += gj * (A)p0[2 * PADW + j];
+                acc3 += gj * (A)p0[3 * PADW + j];
             }
-            x[0][k] = ic_row0 ? T((double)ic - acc0) : T(acc0);
+            x[0][k] = ic_row0 ? T((A)ic - acc0) : T(acc0);
             x[1][k] = T(acc1);
             x[2][k] = T(acc2);
             x[3][k] = T(acc3);
@@ -381,37 +383,40 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
                     for (int s = 0; s < 4; s++) if (z) D[k][s][p] = T(0);
                 }
             }
-            double wsig[NLPL][4], rs[NLPL][4];
+            typedef typename AccT<T>::type A;
+            A wsig[NLPL][4], rs[NLPL][4];
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
                     double w = c->weight[s];
-                    wsig[k][s] = w / sg;
-                    rs[k][s] = (w * ((double)S[s][k] - (double)obs[s][k])) / sg;
+                    wsig[k][s] = A(w / sg);
+                    rs[k][s] = A((w * ((double)S[s][k] - (double)obs[s][k])) / sg);
                 }
             }
 #pragma unroll
             for (int i = 0; i < NT; i++) {
 #pragma unroll
                 for (int h = 0; h <= i; h++) {
-                    double v = 0;
+                    A v = 0;
 #pragma unroll
                     for (int k = 0; k < NLPL; k++)
 #pragma unroll
                         for (int s = 0; s < 4; s++)
-                            v += ((double)D[k][s][i] * (double)D[k][s][h]) * wsig[k][s];
-                    v = group_sum<W>(v);
-                    if (t == 0) ab[i * (i + 1) / 2 + h] = v;
+                            v += ((A)D[k][s][i] * (A)D[k][s][h]) * wsig[k][s];
+                    double sv = sizeof(A) == 8 ? group_sum<W>((double)v)
+                                               : group_sum_comp<W>((float)v);
+                    if (t == 0) ab[i * (i + 1) / 2 + h] = sv;
                 }
-                double b = 0;
+                A b = 0;
 #pragma unroll
                 for (int k = 0; k < NLPL; k++)
 #pragma unroll
-                    for (int s = 0; s < 4; s++) b += rs[k][s] * (double)D[k][s][i];
-                b = group_sum<W>(b);
-                if (t == 0) ab[NP + i] = b;
+                    for (int s = 0; s < 4; s++) b += rs[k][s] * (A)D[k][s][i];
+                double sb = sizeof(A) == 8 ? group_sum<W>((double)b)
+                                           : group_sum_comp<W>((float)b);
+                if (t == 0) ab[NP + i] = sb;
             }
             wave_sync();
         }
@@ -715,13 +720,17 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
 }
 
 // Lanes per pixel for L wavelengths: the smallest power of two >= L up to a
-// full wavefront, then two samples per lane.
-#define PM_DISPATCH_L(L, FN, ...)                                             \
-    ((L) <= 8 ? FN<double, 8, 1, 9>(__VA_ARGS__) :                            \
-     (L) <= 16 ? FN<double, 16, 1, 9>(__VA_ARGS__) :                          \
-     (L) <= 32 ? FN<double, 32, 1, 9>(__VA_ARGS__) :                          \
-     (L) <= 64 ? FN<double, 64, 1, 9>(__VA_ARGS__) :                          \
-                 FN<double, 64, 2, 9>(__VA_ARGS__))
+// full wavefront, then two samples per lane.  T = double (FP64 form) or float
+// (mixed form: spectra, RFs and convolution in FP32).
+#define PM_DISPATCH_LT(T, L, FN, ...)                                         \
+    ((L) <= 8 ? FN<T, 8, 1, 9>(__VA_ARGS__) :                                 \
+     (L) <= 16 ? FN<T, 16, 1, 9>(__VA_ARGS__) :                               \
+     (L) <= 32 ? FN<T, 32, 1, 9>(__VA_ARGS__) :                               \
+     (L) <= 64 ? FN<T, 64, 1, 9>(__VA_ARGS__) :                               \
+                 FN<T, 64, 2, 9>(__VA_ARGS__))
+#define PM_DISPATCH_L(ctx, L, FN, ...)                                        \
+    ((ctx)->precision == PM_MIXED ? PM_DISPATCH_LT(float, L, FN, __VA_ARGS__) \
+                                  : PM_DISPATCH_LT(double, L, FN, __VA_ARGS__))
 
 extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectra,
                                double *d_rf, void *stream)
@@ -731,7 +740,7 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     hipStream_t st = (hipStream_t)stream;
     ctx->last_stream = st;
     const int L = ctx->h.L;
-    return PM_DISPATCH_L(L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
+    return PM_DISPATCH_L(ctx, L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
 }
 
 extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
@@ -756,7 +765,7 @@ extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, do
     a.counters = ctx->counters;
     a.fallbacks = ctx->fallbacks;
     const int L = ctx->h.L;
-    return PM_DISPATCH_L(L, launch_invert, ctx, a, st);
+    return PM_DISPATCH_L(ctx, L, launch_invert, ctx, a, st);
 }
 
 extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
