@@ -31,23 +31,73 @@ struct PmConst {
     int miter, logclambda;
     int fix[11];
     double lambda0, geff;
-    double noise, ilambda, toplim, mu;
+    double noise, inv_noise, ilambda, toplim, mu;
     double weight[4];
     double guess[11];
     double nup[PM_MAXC], wep[PM_MAXC];
     double nub[PM_MAXC], web[PM_MAXC], wer[PM_MAXC];
     double lam[PM_LMAX];
+    double dlam[PM_LMAX];   // lam - lambda0 in double (mil_sinrf.c:124 subtracts in double)
     double G[PM_LMAX];
 };
 
 // ------------------------------------------------------------ group helpers
 
+// Cross-lane moves inside a group.  Within a row of 16 lanes DPP
+// quad_perm [1,0,3,2] / [2,3,0,1], row_half_mirror and row_mirror pair every
+// lane with a lane of the other half at each level; rows are paired by a
+// ds_swizzle xor-16 inside each 32-lane half; the two halves of a wave by
+// readlane.  After each level all lanes of the level's span hold the same
+// partial sum, so every lane of the group ends with an identical total.
+template <int CTRL>
+__device__ __forceinline__ int dpp_i(int v)
+{
+    return __builtin_amdgcn_mov_dpp(v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(dpp_i<CTRL>(hi), dpp_i<CTRL>(lo));
+}
+__device__ __forceinline__ double swz16_d(double v)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_ds_swizzle(hi, 0x401F),
+                            __builtin_amdgcn_ds_swizzle(lo, 0x401F));
+}
+__device__ __forceinline__ double readlane_d(double v, int lane)
+{
+    int lo = __double2loint(v), hi = __double2hiint(v);
+    return __hiloint2double(__builtin_amdgcn_readlane(hi, lane),
+                            __builtin_amdgcn_readlane(lo, lane));
+}
+
 template <int W>
 __device__ __forceinline__ double group_sum(double v)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    v += dpp_d<0xB1>(v);                 // quad_perm [1,0,3,2]
+    v += dpp_d<0x4E>(v);                 // quad_perm [2,3,0,1]
+    v += dpp_d<0x141>(v);                // row_half_mirror
+    if (W >= 16) v += dpp_d<0x140>(v);   // row_mirror
+    if (W >= 32) v += swz16_d(v);        // rows 0<->1, 2<->3
+    if (W >= 64) v = readlane_d(v, 0) + readlane_d(v, 32);
     return v;
+}
+
+template <int W>
+__device__ __forceinline__ float group_xchg_f(float v, int level)
+{
+    int i = __float_as_int(v);
+    switch (level) {
+    case 0: i = dpp_i<0xB1>(i); break;
+    case 1: i = dpp_i<0x4E>(i); break;
+    case 2: i = dpp_i<0x141>(i); break;
+    case 3: i = dpp_i<0x140>(i); break;
+    default: i = __builtin_amdgcn_ds_swizzle(i, 0x401F); break;
+    }
+    return __int_as_float(i);
 }
 
 // Compensated FP32 group sum: a TwoSum butterfly carrying the rounding
@@ -57,18 +107,21 @@ __device__ __forceinline__ double group_sum(double v)
 template <int W>
 __device__ __forceinline__ double group_sum_comp(float v)
 {
+    constexpr int LEVELS = W == 8 ? 3 : W == 16 ? 4 : 5;
     float sv = v, e = 0.0f;
 #pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) {
-        float so = __shfl_xor(sv, o, W);
-        float eo = __shfl_xor(e, o, W);
+    for (int lv = 0; lv < LEVELS; lv++) {
+        float so = group_xchg_f<W>(sv, lv);
+        float eo = group_xchg_f<W>(e, lv);
         float t = __fadd_rn(sv, so);
         float bp = __fsub_rn(t, sv);
         float err = __fadd_rn(__fsub_rn(sv, __fsub_rn(t, bp)), __fsub_rn(so, bp));
         sv = t;
         e = __fadd_rn(__fadd_rn(e, eo), err);
     }
-    return (double)sv + (double)e;
+    double r = (double)sv + (double)e;
+    if (W >= 64) r = readlane_d(r, 0) + readlane_d(r, 32);
+    return r;
 }
 
 template <typename T> struct AccT { typedef double type; };
@@ -122,9 +175,14 @@ __device__ __forceinline__ void voigt(T a, T v, T &H, T &F)
     PM_HORNER(dr, di, B5) PM_HORNER(dr, di, B4) PM_HORNER(dr, di, B3)
     PM_HORNER(dr, di, B2) PM_HORNER(dr, di, B1) PM_HORNER(dr, di, B0)
 #undef PM_HORNER
-    // w = n / d (Smith's scaling keeps the FP32 path in range)
+    // w = n / d: one reciprocal of |d|^2 in FP64; Smith's scaling in FP32,
+    // where |d|^2 would overflow for |v| beyond ~2000
     T wr, wi;
-    if (fabs(dr) >= fabs(di)) {
+    if (sizeof(T) == 8) {
+        T inv = T(1) / (dr * dr + di * di);
+        wr = (nr * dr + ni * di) * inv;
+        wi = (ni * dr - nr * di) * inv;
+    } else if (fabs(dr) >= fabs(di)) {
         T r = di / dr, den = dr + di * r;
         T inv = T(1) / den;
         wr = (nr + ni * r) * inv;
@@ -147,6 +205,7 @@ struct Geom {
     T sin_gm, cosi, sinis, cosis, cosis_2, sina, cosa;
     T sinda, cosda, sindi, cosdi, sinis_cosa, sinis_sina;
     T E0, LD, A, MF, B0, B1, ulos, parcial;
+    T invE0, invLD;
 };
 
 template <typename T>
@@ -155,9 +214,10 @@ __device__ __forceinline__ void make_geom(const PmConst *__restrict__ c, const d
 {
     const double CC = PM_PI / 180.0, CC_2 = CC * 2;      // milos.c:232-233
     double GM = m[5] * CC, AZI = m[6] * CC;
-    double sgm = sin(GM), cgm = cos(GM);
+    double sgm, cgm, sa, ca;
+    sincos(GM, &sgm, &cgm);
     double azi_2 = 2 * AZI;
-    double sa = sin(azi_2), ca = cos(azi_2);
+    sincos(azi_2, &sa, &ca);
     double sinis = sgm * sgm, cosis = cgm * cgm;
     g.sin_gm = T(sgm); g.cosi = T(cgm);
     g.sinis = T(sinis); g.cosis = T(cosis); g.cosis_2 = T((1 + cosis) / 2);
@@ -167,6 +227,8 @@ __device__ __forceinline__ void make_geom(const PmConst *__restrict__ c, const d
     g.sinis_cosa = T(sinis * ca); g.sinis_sina = T(sinis * sa);
     g.E0 = T(m[0]);                                        // FO = 1
     g.LD = T(m[3]); g.A = T(m[4]); g.MF = T(m[1]);
+    g.invE0 = T(1.0 / m[0]);
+    g.invLD = T(1.0 / m[3]);
     g.B0 = T(m[7]); g.B1 = T(-(m[8] * c->mu));
     g.ulos = T((m[2] * c->lambda0) / (PM_VLIGHT * m[3]));      // mil_sinrf.c:120
     g.parcial = T(((c->lambda0 * c->lambda0) / m[3]) * PM_CTE4_6_13);  // :153
@@ -184,9 +246,11 @@ struct Samp {
 // Components and profile sums at one wavelength, mil_sinrf.c:122-183,418-449.
 template <typename T>
 __device__ __forceinline__ void profiles(const PmConst *__restrict__ c, const Geom<T> &g,
-                                         T lam, Samp<T> &s)
+                                         double dl, Samp<T> &s)
 {
-    T u = ((lam - T(c->lambda0)) / g.LD) - g.ulos;
+    // the wavelength offset is formed in double: in FP32 the 0.5 mA spacing
+    // of floats near 6000 A would be a percent of a Doppler width
+    T u = (T(dl) * g.invLD) - g.ulos;
     // pi components
     {
         T fi = 0, shi = 0;
@@ -298,7 +362,7 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
     // dshi[q][j] = dF_u[q] * f[q][j] for j = B, vlos, dopp (f = -nux, auxCte,
     // -uu/LD); the damping row uses -2 dF_u and dH_u / 2 (me_der.c:581-600)
     T dH_u[3], dF_u[3], fB[3], fD[3];
-    const T auxCte = T(-c->lambda0) / (T(PM_VLIGHT) * LD);
+    const T auxCte = T(-c->lambda0 / PM_VLIGHT) * g.invLD;
 #pragma unroll
     for (int q = 0; q < 3; q++) {
         T w, nux;
@@ -315,7 +379,7 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
         dH_u[q] = ((T(4) * A * s.F[q]) - (T(2) * s.uu[q] * s.H[q])) * w;
         dF_u[q] = (T(PM_RR) - A * s.H[q] - T(2) * s.uu[q] * s.F[q]) * w;
         fB[q] = -nux;
-        fD[q] = -s.uu[q] / LD;
+        fD[q] = -s.uu[q] * g.invLD;
     }
     const T E0_2 = E0 / T(2);
     const T cosi_E0_2 = E0_2 * g.cosi;
@@ -334,7 +398,7 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
     const T ei = r.etai, eq = r.etaq, eu = r.etau, ev = r.etav;
     const T rq = r.rhoq, ru = r.rhou, rv = r.rhov;
     const T ei2 = r.etai_2, gp1 = r.gp1, gp2 = r.gp2, gp3 = r.gp3, dt = r.dt;
-    const T dtaux = (g.B1) / (dt * dt);
+    const T dtaux = (g.B1) * (r.dti * r.dti);
     const T etai_gp3 = ei * gp3;
     const T aux2 = T(2) * ei;
     const T ext1 = aux2 * eq + ev * ru - eu * rv;
@@ -347,13 +411,13 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
         // (me_der.c:93-199)
         T a_ei, a_eq, a_eu, a_ev, a_rq, a_ru, a_rv;
         if (il == 0) {
-            a_ei = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2)) / E0;
-            a_eq = (r.parcial2 * g.sinis_cosa) / E0;
-            a_eu = (r.parcial2 * g.sinis_sina) / E0;
-            a_ev = ((s.fi[2] - s.fi[1]) * cosi_E0_2) / E0;
-            a_rq = (r.parcial3 * g.sinis_cosa) / E0;
-            a_ru = (r.parcial3 * g.sinis_sina) / E0;
-            a_rv = ((s.shi[2] - s.shi[1]) * cosi_E0_2) / E0;
+            a_ei = ((E0_2) * (s.fi[0] * g.sinis + (r.parcial1) * g.cosis_2)) * g.invE0;
+            a_eq = (r.parcial2 * g.sinis_cosa) * g.invE0;
+            a_eu = (r.parcial2 * g.sinis_sina) * g.invE0;
+            a_ev = ((s.fi[2] - s.fi[1]) * cosi_E0_2) * g.invE0;
+            a_rq = (r.parcial3 * g.sinis_cosa) * g.invE0;
+            a_ru = (r.parcial3 * g.sinis_sina) * g.invE0;
+            a_rv = ((s.shi[2] - s.shi[1]) * cosi_E0_2) * g.invE0;
         } else if (il <= 4) {
             T fp, fb, fr, hp, hb, hr;
             if (il == 4) {                                       // damping
@@ -499,7 +563,7 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
 {
     constexpr int P = N * (N + 1) / 2;
     // F holds L (strictly lower, unit diagonal implied) with D on the diagonal
-    double F[P];
+    double F[P], id[N];
     bool act[N];
 #pragma unroll
     for (int i = 0; i < N; i++) {
@@ -514,6 +578,7 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         const int jj = j * (j + 1) / 2;
         if (!act[j]) {
             F[jj + j] = 1.0;
+            id[j] = 1.0;
 #pragma unroll
             for (int k = 0; k < j; k++) F[jj + k] = 0.0;
 #pragma unroll
@@ -525,13 +590,15 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         for (int k = 0; k < j; k++) dj -= F[jj + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
         if (!(dj > 0.0)) return false;
         F[jj + j] = dj;
+        const double inv_dj = 1.0 / dj;
+        id[j] = inv_dj;
 #pragma unroll
         for (int i = j + 1; i < N; i++) {
             const int ii = i * (i + 1) / 2;
             double v = F[ii + j];
 #pragma unroll
             for (int k = 0; k < j; k++) v -= F[ii + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
-            F[ii + j] = act[i] ? v / dj : 0.0;
+            F[ii + j] = act[i] ? v * inv_dj : 0.0;
         }
     }
     // solve L D L^T x = beta
@@ -544,7 +611,7 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         x[i] = v;
     }
 #pragma unroll
-    for (int i = 0; i < N; i++) x[i] /= F[i * (i + 1) / 2 + i];
+    for (int i = 0; i < N; i++) x[i] *= id[i];
 #pragma unroll
     for (int i = N - 1; i >= 0; i--) {
         double v = x[i];
@@ -574,18 +641,23 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         double s = 1.0;
 #pragma unroll
         for (int i = 0; i < k; i++) s += F[kk + i] * F[kk + i];
-        tr += s / F[kk + k];
+        tr += s * id[k];
     }
     return tr < 1e9;
 }
 
 // Eigen pseudo-inverse by cyclic Jacobi rotations on a matrix kept in LDS,
 // the lanes of the group sharing the row/column updates.  Same rotation
-// formula, sweep order and stopping rule as the CPU restatement.  A: N*N in
-// LDS (overwritten), V: N*N in LDS; beta, delta in registers (uniform).
+// formula, sweep order and stopping rule as the CPU restatement.  ws (LDS):
+// A = ws[0..N*N) (damped matrix, overwritten), V = ws[N*N..2N*N),
+// beta = ws[2N*N..+N), delta written to ws[2N*N+N..+N).  Not inlined: it is
+// the rare path and keeps its registers out of the main loop.
 template <int W, int N>
-__device__ void jacobi_pinv(double *A, double *V, int t, const double *beta, double *delta)
+__device__ __noinline__ void jacobi_pinv(double *ws, int t)
 {
+    double *A = ws, *V = ws + N * N;
+    const double *beta = ws + 2 * N * N;
+    double *delta = ws + 2 * N * N + N;
     for (int i = t; i < N * N; i += W) V[i] = ((i / N) == (i % N)) ? 1.0 : 0.0;
     wave_sync();
     for (int sweep = 0; sweep < 100; sweep++) {
@@ -626,16 +698,15 @@ __device__ void jacobi_pinv(double *A, double *V, int t, const double *beta, dou
             }
         }
     }
-    double aux[N];
-    for (int j = 0; j < N; j++) {
+    // delta_i = sum_j V_ij (V^T beta)_j / w_j, |w_j| > 1e-12 (milosUtils.c:254-269)
+    for (int i = t; i < N; i += W) {
         double s = 0;
-        for (int i = 0; i < N; i++) s += beta[i] * V[i * N + j];
-        double w = A[j * N + j];
-        aux[j] = s * ((fabs(w) > 1e-12) ? (1 / w) : 0.0);
-    }
-    for (int i = 0; i < N; i++) {
-        double s = 0;
-        for (int j = 0; j < N; j++) s += V[i * N + j] * aux[j];
+        for (int j = 0; j < N; j++) {
+            double a = 0;
+            for (int k = 0; k < N; k++) a += beta[k] * V[k * N + j];
+            double w = A[j * N + j];
+            s += V[i * N + j] * (a * ((fabs(w) > 1e-12) ? (1 / w) : 0.0));
+        }
         delta[i] = s;
     }
     wave_sync();
@@ -643,10 +714,15 @@ __device__ void jacobi_pinv(double *A, double *V, int t, const double *beta, dou
 
 // ------------------------------------------------------------ classical estimates
 
+struct Classic {
+    double B, vlos, gm, az;
+};
+
 // estimacionesClasicas(forInitialUse = 1), milosUtils.c:287-452, on the
 // observed profile staged in LDS (each lane runs it redundantly, in the
-// reference's summation order).  Writes B, vlos, gm, az of m.
-__device__ inline void classical(const PmConst *__restrict__ c, const float *spec, double *m)
+// reference's summation order), followed by the NaN guards of the pixel loop.
+// Not inlined: it runs once per pixel.
+__device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const float *spec)
 {
     const int n = c->L;
     const double l0 = c->lambda0;
@@ -718,13 +794,15 @@ __device__ inline void classical(const PmConst *__restrict__ c, const float *spe
     double B_aux = fabs(Blos / cos(gamma_rad));
     if (Vlos < (-4)) Vlos = -4;
     if (Vlos > (4)) Vlos = 4;
-    m[1] = (B_aux > 4000 ? 4000 : B_aux);
-    m[2] = Vlos;
-    m[5] = gamma;
-    m[6] = phi;
+    Classic r;
+    r.B = (B_aux > 4000 ? 4000 : B_aux);
+    r.vlos = Vlos;
+    r.gm = gamma;
+    r.az = phi;
     // NaN guards of the pixel loop, milos.c:976-983
-    if (isnan(m[1])) m[1] = 1;
-    if (isnan(m[2])) m[2] = 1e-3;
-    if (isnan(m[5])) m[5] = 1;
-    if (isnan(m[6])) m[6] = 1;
+    if (isnan(r.B)) r.B = 1;
+    if (isnan(r.vlos)) r.vlos = 1e-3;
+    if (isnan(r.gm)) r.gm = 1;
+    if (isnan(r.az)) r.az = 1;
+    return r;
 }

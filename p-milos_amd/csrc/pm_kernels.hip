@@ -30,13 +30,14 @@ template <typename T, int W, int NLPL, int NT>
 struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
     static constexpr int PAD_BYTES = 4 * PADW * (int)sizeof(T);
-    static constexpr int EIG_BYTES = 2 * NT * NT * 8;
+    static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
     static constexpr int U_BYTES =
         ((PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES) + 15) / 16 * 16;
     static constexpr int OBS_BYTES = (4 * W * NLPL * 4 + 15) / 16 * 16;
     static constexpr int NP = NT * (NT + 1) / 2;
     static constexpr int AB_BYTES = ((NP + NT) * 8 + 15) / 16 * 16;
-    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES;
+    static constexpr int LM_BYTES = 24 * 8;                   // model[11], trial[11]
+    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES;
     static constexpr int GPB = PM_BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
 };
@@ -120,7 +121,7 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         int li = lam < L ? lam : L - 1;
-        profiles<T>(c, g, T(c->lam[li]), sp[k]);
+        profiles<T>(c, g, c->dlam[li], sp[k]);
         UR<T> r;
         unno<T>(g, sp[k], r);
         T s4[4];
@@ -211,7 +212,7 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
 // loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
 template <typename T, int W, int NLPL, int NT>
-__global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
+__global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kernel(InvArgs a)
 {
     using LD = Lds<T, W, NLPL, NT>;
     constexpr int NP = LD::NP;
@@ -223,13 +224,15 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
     T *pad = (T *)gb;
     float *sobs = (float *)(gb + LD::U_BYTES);
     double *ab = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES);
+    // the group-uniform model (lm[0..10]) and trial model (lm[11..21]) live
+    // in LDS; lane 0 writes them
+    double *lm = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES);
     for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
 
     const int xcd = blockIdx.x % PM_NSLICE;
     int sl_off = 0, cur = 0, end = 0;
 
-    double model[11], trial[11];
     double flambda = 0, ochisqr = 0, chisqr0 = 0, factor = 1.0;
     int it = 0, nacc = 0, nfree = 1, pix = -1;
     bool iterating = false;
@@ -269,13 +272,22 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             }
             wave_sync();
             nfree = 4 * L - NT - group_sum_int<W>(ng);        // :507-508
-#pragma unroll
-            for (int i = 0; i < 11; i++) model[i] = c->guess[i];
-            classical(c, sobs, model);
-            wave_sync();
+            {
+                Classic e = classical(c, sobs);
+                wave_sync();
+                if (t == 0) {
+                    for (int i = 0; i < 11; i++) lm[i] = c->guess[i];
+                    lm[1] = e.B;
+                    lm[2] = e.vlos;
+                    lm[5] = e.gm;
+                    lm[6] = e.az;
+                    for (int i = 0; i < 11; i++) lm[11 + i] = lm[i];
+                }
+                wave_sync();
+            }
             if (nfree == 0) {                                 // :509-512, milos.c:997
                 if (t == 0) {
-                    for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = model[i];
+                    for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = lm[i];
                     a.chi2[pix] = 0.0f;
                     a.iters[pix] = pix;
                     if (a.accepted) a.accepted[pix] = 0;
@@ -294,36 +306,44 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             it = 0;
             nacc = 0;
             factor = 1.0;
-#pragma unroll
-            for (int i = 0; i < 11; i++) trial[i] = model[i];
             first = true;
         } else {
             // covar = alpha with the diagonal scaled by (1 + lambda), :559-563
             double delta[NT];
             bool ok = ldl_solve<NT>(ab, 1.0 + flambda, delta);
             if (!ok) {
-                double bb[NT];
-#pragma unroll
-                for (int i = 0; i < NT; i++) bb[i] = ab[NP + i];
-                double *EA = (double *)pad, *EV = EA + NT * NT;
+                double *ws = (double *)pad;
                 for (int e = t; e < NT * NT; e += W) {
                     int i = e / NT, j = e % NT;
                     int ii = i > j ? i : j, jj = i > j ? j : i;
                     double v = ab[ii * (ii + 1) / 2 + jj];
-                    EA[e] = (i == j) ? v * (1.0 + flambda) : v;
+                    ws[e] = (i == j) ? v * (1.0 + flambda) : v;
                 }
+                for (int i = t; i < NT; i += W) ws[2 * NT * NT + i] = ab[NP + i];
                 wave_sync();
-                jacobi_pinv<W, NT>(EA, EV, t, bb, delta);
+                jacobi_pinv<W, NT>(ws, t);
+#pragma unroll
+                for (int i = 0; i < NT; i++) delta[i] = ws[2 * NT * NT + NT + i];
+                wave_sync();
                 for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
                 wave_sync();
                 if (t == 0) atomicAdd(a.fallbacks, 1ull);
             }
-            aplica_delta(c, NT, model, delta, trial);         // :566
-            check_model(trial);                               // :568
+            {
+                double cur[11], tr[11];
+#pragma unroll
+                for (int i = 0; i < 11; i++) { cur[i] = lm[i]; tr[i] = lm[11 + i]; }
+                aplica_delta(c, NT, cur, delta, tr);          // :566
+                check_model(tr);                              // :568
+                wave_sync();
+                if (t == 0)
+                    for (int i = 0; i < 11; i++) lm[11 + i] = tr[i];
+                wave_sync();
+            }
         }
 
         // trial synthesis and chi2, :569-571 (fchisqr, lib.c:126-152)
-        synth_slots<T, W, NLPL>(c, trial, pad, t, L, g, sp, S);
+        synth_slots<T, W, NLPL>(c, lm + 11, pad, t, L, g, sp, S);
         double part = 0;
 #pragma unroll
         for (int k = 0; k < NLPL; k++) {
@@ -331,9 +351,9 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             if (lam < L) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
+                    double isg = obs[s][k] <= -1.0f ? 1e-21 : c->inv_noise;
                     double d = (double)S[s][k] - (double)obs[s][k];
-                    part += (((d * d) * c->weight[s]) / sg);
+                    part += (((d * d) * c->weight[s]) * isg);
                 }
             }
         }
@@ -350,8 +370,9 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             bool acc = (c2 - ochisqr < 0.);
             if (acc) {                                         // :581-606
                 flambda = flambda / (5.0 * factor);
-#pragma unroll
-                for (int i = 0; i < 11; i++) model[i] = trial[i];
+                if (t == 0)
+                    for (int i = 0; i < 11; i++) lm[i] = lm[11 + i];
+                wave_sync();
                 ochisqr = c2;
                 nacc++;
             } else {                                           // :607-615
@@ -389,10 +410,10 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
             for (int k = 0; k < NLPL; k++) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    double sg = obs[s][k] <= -1.0f ? 1e21 : c->noise;
+                    double isg = obs[s][k] <= -1.0f ? 1e-21 : c->inv_noise;
                     double w = c->weight[s];
-                    wsig[k][s] = A(w / sg);
-                    rs[k][s] = A((w * ((double)S[s][k] - (double)obs[s][k])) / sg);
+                    wsig[k][s] = A(w * isg);
+                    rs[k][s] = A((w * ((double)S[s][k] - (double)obs[s][k])) * isg);
                 }
             }
 #pragma unroll
@@ -423,7 +444,7 @@ __global__ void __launch_bounds__(PM_BLOCK) pm_invert_kernel(InvArgs a)
 
         if (finished) {
             if (t == 0) {
-                for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = model[i];
+                for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = lm[i];
                 a.chi2[pix] = (float)ochisqr;                   // :625
                 a.iters[pix] = it;
                 if (a.accepted) a.accepted[pix] = nacc;
@@ -645,12 +666,14 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     for (int i = 0; i < 11; i++) { c.fix[i] = cfg->fix[i]; c.guess[i] = cfg->guess[i]; }
     c.lambda0 = cfg->lambda0;
     c.noise = cfg->noise;
+    c.inv_noise = 1.0 / cfg->noise;
     c.ilambda = cfg->ilambda;
     c.toplim = cfg->toplim;
     c.mu = cfg->mu;
     for (int i = 0; i < 4; i++) c.weight[i] = cfg->weight[i];
     for (int i = 0; i < L; i++) {
         c.lam[i] = cfg->lambda[i];
+        c.dlam[i] = cfg->lambda[i] - cfg->lambda0;
         c.G[i] = cfg->psf ? cfg->psf[i] : 0.0;
     }
     ctx->precision = cfg->precision;
