@@ -39,6 +39,9 @@ struct PmConst {
     double lam[PM_LMAX];
     double dlam[PM_LMAX];   // lam - lambda0 in double (mil_sinrf.c:124 subtracts in double)
     double G[PM_LMAX];
+    // PSF taps [glo, ghi) whose weight reaches 1e-17 (FP64 form) / 1e-9
+    // (FP32 form) of the peak: the others are below the rounding of the sum
+    int glo64, ghi64, glo32, ghi32;
 };
 
 // ------------------------------------------------------------ group helpers

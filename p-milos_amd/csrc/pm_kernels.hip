@@ -31,13 +31,16 @@ struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
     static constexpr int PAD_BYTES = 4 * PADW * (int)sizeof(T);
     static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
+    // J^T J cross-lane reduction: CH values per chunk, double per lane
+    static constexpr int CH = W >= 16 ? 16 : W;
+    static constexpr int RED_BYTES = CH * W * 8;
     static constexpr int U_BYTES =
         ((PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES) + 15) / 16 * 16;
     static constexpr int OBS_BYTES = (4 * W * NLPL * 4 + 15) / 16 * 16;
     static constexpr int NP = NT * (NT + 1) / 2;
     static constexpr int AB_BYTES = ((NP + NT) * 8 + 15) / 16 * 16;
     static constexpr int LM_BYTES = 24 * 8;                   // model[11], trial[11]
-    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES;
+    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES + RED_BYTES;
     static constexpr int GPB = PM_BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
 };
@@ -84,7 +87,9 @@ __device__ __forceinline__ void conv4(const PmConst *__restrict__ c, T *pad, int
         if (lam < L) {
             A acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
             const T *p0 = pad + lam;
-            for (int j = 0; j < L; j++) {
+            const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
+            const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
+            for (int j = jlo; j < jhi; j++) {
                 A gj = A(c->G[j]);
                 acc0 += gj * (A)p0[j];
                 acc1 += gj * (A)p0[PADW + j];
@@ -227,6 +232,8 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
     // the group-uniform model (lm[0..10]) and trial model (lm[11..21]) live
     // in LDS; lane 0 writes them
     double *lm = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES);
+    // J^T J reduction buffer (separate: the PSF row buffer keeps zero margins)
+    double *red = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
     for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
 
@@ -416,28 +423,49 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
                     rs[k][s] = A((w * ((double)S[s][k] - (double)obs[s][k])) * isg);
                 }
             }
+            // The NP + NT sums are reduced CH at a time through LDS: every lane
+            // stores its partials, then lane t adds entry t % CH over its
+            // share of the lanes (the shares combined by a cross-row swizzle /
+            // shuffle) and one lane per entry writes it for the solve.
+            constexpr int CH = LD::CH, NV = NP + NT, NCH = (NV + CH - 1) / CH;
+            constexpr int PARTS = W / CH;
 #pragma unroll
-            for (int i = 0; i < NT; i++) {
+            for (int ch = 0; ch < NCH; ch++) {
 #pragma unroll
-                for (int h = 0; h <= i; h++) {
-                    A v = 0;
+                for (int i = 0; i < NT; i++) {
 #pragma unroll
-                    for (int k = 0; k < NLPL; k++)
+                    for (int h = 0; h <= i; h++) {
+                        const int idx = i * (i + 1) / 2 + h;
+                        if (idx / CH != ch) continue;
+                        A v = 0;
 #pragma unroll
-                        for (int s = 0; s < 4; s++)
-                            v += ((A)D[k][s][i] * (A)D[k][s][h]) * wsig[k][s];
-                    double sv = sizeof(A) == 8 ? group_sum<W>((double)v)
-                                               : group_sum_comp<W>((float)v);
-                    if (t == 0) ab[i * (i + 1) / 2 + h] = sv;
+                        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                            for (int s = 0; s < 4; s++)
+                                v += ((A)D[k][s][i] * (A)D[k][s][h]) * wsig[k][s];
+                        red[t * CH + idx % CH] = (double)v;
+                    }
+                    const int ib = NP + i;
+                    if (ib / CH == ch) {
+                        A b = 0;
+#pragma unroll
+                        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                            for (int s = 0; s < 4; s++) b += rs[k][s] * (A)D[k][s][i];
+                        red[t * CH + ib % CH] = (double)b;
+                    }
                 }
-                A b = 0;
+                wave_sync();
+                const int vv = t % CH, part = t / CH;
+                constexpr int SPAN = W / PARTS;
+                double sum = 0;
 #pragma unroll
-                for (int k = 0; k < NLPL; k++)
-#pragma unroll
-                    for (int s = 0; s < 4; s++) b += rs[k][s] * (A)D[k][s][i];
-                double sb = sizeof(A) == 8 ? group_sum<W>((double)b)
-                                           : group_sum_comp<W>((float)b);
-                if (t == 0) ab[NP + i] = sb;
+                for (int l = 0; l < SPAN; l++) sum += red[(part * SPAN + l) * CH + vv];
+                if (PARTS >= 2) sum += swz16_d(sum);
+                if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
+                const int e = ch * CH + vv;
+                if (part == 0 && e < NV) ab[e] = sum;
+                wave_sync();
             }
             wave_sync();
         }
@@ -675,6 +703,19 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
         c.lam[i] = cfg->lambda[i];
         c.dlam[i] = cfg->lambda[i] - cfg->lambda0;
         c.G[i] = cfg->psf ? cfg->psf[i] : 0.0;
+    }
+    // PSF tap ranges per form (see PmConst)
+    {
+        double gmax = 0;
+        for (int i = 0; i < L; i++) gmax = fmax(gmax, fabs(c.G[i]));
+        int lo64 = L, hi64 = 0, lo32 = L, hi32 = 0;
+        for (int i = 0; i < L; i++) {
+            double r = gmax > 0 ? fabs(c.G[i]) / gmax : 0;
+            if (r >= 1e-17) { lo64 = lo64 < i ? lo64 : i; hi64 = i + 1; }
+            if (r >= 1e-9) { lo32 = lo32 < i ? lo32 : i; hi32 = i + 1; }
+        }
+        if (!c.use_psf) { lo64 = lo32 = 0; hi64 = hi32 = 0; }
+        c.glo64 = lo64; c.ghi64 = hi64; c.glo32 = lo32; c.ghi32 = hi32;
     }
     ctx->precision = cfg->precision;
     if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return PM_EDEVICE; }
