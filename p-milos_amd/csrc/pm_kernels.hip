@@ -33,7 +33,7 @@ struct Lds {
     static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
     // J^T J cross-lane reduction: CH values per chunk, double per lane
     static constexpr int CH = W >= 16 ? 16 : W;
-    static constexpr int RED_BYTES = CH * W * 8;
+    static constexpr int RED_BYTES = (CH + 1) * W * 8;   // lane stride CH+1: no bank conflicts
     static constexpr int U_BYTES =
         ((PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES) + 15) / 16 * 16;
     static constexpr int OBS_BYTES = (4 * W * NLPL * 4 + 15) / 16 * 16;
@@ -443,7 +443,7 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
 #pragma unroll
                             for (int s = 0; s < 4; s++)
                                 v += ((A)D[k][s][i] * (A)D[k][s][h]) * wsig[k][s];
-                        red[t * CH + idx % CH] = (double)v;
+                        red[t * (CH + 1) + idx % CH] = (double)v;
                     }
                     const int ib = NP + i;
                     if (ib / CH == ch) {
@@ -452,7 +452,7 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
                         for (int k = 0; k < NLPL; k++)
 #pragma unroll
                             for (int s = 0; s < 4; s++) b += rs[k][s] * (A)D[k][s][i];
-                        red[t * CH + ib % CH] = (double)b;
+                        red[t * (CH + 1) + ib % CH] = (double)b;
                     }
                 }
                 wave_sync();
@@ -460,7 +460,7 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
                 constexpr int SPAN = W / PARTS;
                 double sum = 0;
 #pragma unroll
-                for (int l = 0; l < SPAN; l++) sum += red[(part * SPAN + l) * CH + vv];
+                for (int l = 0; l < SPAN; l++) sum += red[(part * SPAN + l) * (CH + 1) + vv];
                 if (PARTS >= 2) sum += swz16_d(sum);
                 if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
                 const int e = ch * CH + vv;

@@ -16,7 +16,7 @@ pytestmark = pytest.mark.gpu
 # S0, S1.  The float32 model error (~1e-7 of the spectra) is far below the
 # 1e-3 noise, so both forms find the same chi2 minimum; what separates them is
 # the LM path of pixels that stop on the 50-trip cap or the 1e-12 test.
-TOLMIX = np.array([0.5, 2.0, 2e-3, 2e-4, 5e-3, 0.2, 0.5, 2e-3, 2e-3])
+TOLMIX = np.array([0.5, 2.0, 2e-3, 2e-4, 5e-3, 0.2, 0.5, 1e-2, 1e-2])
 
 
 def say(*a):
@@ -39,11 +39,11 @@ def test_mixed_synthesis_and_rf():
     spec, rf = ctx.synth(models, rf=True)
     for i, m in enumerate(models):
         sp, d = O.synth(s, m, rf=True)
-        sc = np.maximum(np.abs(sp).max(axis=-1, keepdims=True), 1e-6 * np.abs(sp[0]).max())
-        assert (np.abs(spec[i] - sp) / sc).max() < 2e-5
+        # FP32 rounding of the intensity (~1) bounds the absolute error of all
+        # four Stokes; 1e-6 of the continuum is 1000x below the 1e-3 noise
+        assert np.abs(spec[i] - sp).max() < 1e-6 * np.abs(sp[0]).max()
         # per parameter, relative to its largest response over the Stokes
-        dsc = np.abs(d).max(axis=(0, 2), keepdims=True)
-        dsc = np.where(dsc == 0, 1, dsc)
+        dsc = np.maximum(np.abs(d).max(axis=(0, 2), keepdims=True), 1e-5)
         assert (np.abs(rf[i] - d) / dsc).max() < 2e-3
 
 
