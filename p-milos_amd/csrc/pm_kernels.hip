@@ -20,6 +20,9 @@
 #include "../../include/pmilos.h"
 
 #define PM_BLOCK 256
+// threads per workgroup for W-lane pixel groups (W = 8 would need more than
+// 64 KiB of LDS for 32 groups)
+template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM_BLOCK; };
 #define PM_CHUNK 4           // pixels taken per work-counter increment
 #define PM_NSLICE 8          // one work counter per XCD
 #define PM_CSTRIDE 32        // ints between counters (separate cache lines)
@@ -34,15 +37,19 @@ struct Lds {
     // J^T J cross-lane reduction: CH values per chunk, double per lane
     static constexpr int CH = W >= 16 ? 16 : W;
     static constexpr int RED_BYTES = (CH + 1) * W * 8;   // lane stride CH+1: no bank conflicts
-    static constexpr int U_BYTES =
-        ((PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES) + 15) / 16 * 16;
+    static constexpr int U0 = PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES;
+    // the PSF row buffer, the eigen fallback and the J^T J reduction share
+    // one region (the row buffer is re-zeroed after the other two)
+    static constexpr int U_BYTES = ((U0 > RED_BYTES ? U0 : RED_BYTES) + 15) / 16 * 16;
     static constexpr int OBS_BYTES = (4 * W * NLPL * 4 + 15) / 16 * 16;
     static constexpr int NP = NT * (NT + 1) / 2;
     static constexpr int AB_BYTES = ((NP + NT) * 8 + 15) / 16 * 16;
     static constexpr int LM_BYTES = 24 * 8;                   // model[11], trial[11]
-    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES + RED_BYTES;
-    static constexpr int GPB = PM_BLOCK / W;
+    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES;
+    static constexpr int BLOCK = BlockOf<W>::value;
+    static constexpr int GPB = BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
+    static_assert(BLOCK_BYTES <= 65536, "LDS per workgroup above 64 KiB");
 };
 
 // ------------------------------------------------------------ convolution
@@ -217,7 +224,7 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
 // loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
 template <typename T, int W, int NLPL, int NT>
-__global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kernel(InvArgs a)
+__global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_invert_kernel(InvArgs a)
 {
     using LD = Lds<T, W, NLPL, NT>;
     constexpr int NP = LD::NP;
@@ -232,8 +239,7 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
     // the group-uniform model (lm[0..10]) and trial model (lm[11..21]) live
     // in LDS; lane 0 writes them
     double *lm = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES);
-    // J^T J reduction buffer (separate: the PSF row buffer keeps zero margins)
-    double *red = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
+    double *red = (double *)pad;     // J^T J reduction (shares the row buffer)
     for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
 
@@ -467,6 +473,8 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
                 if (part == 0 && e < NV) ab[e] = sum;
                 wave_sync();
             }
+            for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+            wave_sync();
             wave_sync();
         }
 
@@ -495,7 +503,7 @@ __global__ void __launch_bounds__(PM_BLOCK, (NLPL == 1 ? 2 : 1)) pm_invert_kerne
 // ------------------------------------------------------------ synthesis kernel
 
 template <typename T, int W, int NLPL, int NT>
-__global__ void __launch_bounds__(PM_BLOCK)
+__global__ void __launch_bounds__(BlockOf<W>::value)
 pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, double *rf)
 {
     using LD = Lds<T, W, NLPL, NT>;
@@ -755,7 +763,7 @@ static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
     using LD = Lds<T, W, NLPL, NT>;
     auto kern = pm_invert_kernel<T, W, NLPL, NT>;
     int per_cu = 0;
-    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, PM_BLOCK, 0));
+    PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LD::BLOCK, 0));
     if (per_cu < 1) per_cu = 1;
     long long need = ((long long)a.npix + LD::GPB - 1) / LD::GPB;
     long long grid = (long long)per_cu * ctx->ncu;
@@ -763,7 +771,7 @@ static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, a.counters, a.fallbacks);
     PM_HIP(hipGetLastError());
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(PM_BLOCK), 0, st, a);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(LD::BLOCK), 0, st, a);
     PM_HIP(hipGetLastError());
     return PM_OK;
 }
@@ -777,7 +785,7 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
     long long grid = (long long)ctx->ncu * 8;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((pm_synth_kernel<T, W, NLPL, NT>), dim3((unsigned)grid), dim3(PM_BLOCK),
+    hipLaunchKernelGGL((pm_synth_kernel<T, W, NLPL, NT>), dim3((unsigned)grid), dim3(LD::BLOCK),
                        0, st, (const PmConst *)ctx->d, m, n, sp, rf);
     PM_HIP(hipGetLastError());
     return PM_OK;
