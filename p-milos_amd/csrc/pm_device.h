@@ -127,6 +127,10 @@ __device__ __forceinline__ double group_sum_comp(float v)
     return r;
 }
 
+// 1/x: IEEE division in FP64; the hardware reciprocal (1 ulp) in FP32
+__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+__device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+
 template <typename T> struct AccT { typedef double type; };
 template <> struct AccT<float> { typedef float type; };
 
@@ -186,13 +190,13 @@ __device__ __forceinline__ void voigt(T a, T v, T &H, T &F)
         wr = (nr * dr + ni * di) * inv;
         wi = (ni * dr - nr * di) * inv;
     } else if (fabs(dr) >= fabs(di)) {
-        T r = di / dr, den = dr + di * r;
-        T inv = T(1) / den;
+        T r = di * recip(dr), den = dr + di * r;
+        T inv = recip(den);
         wr = (nr + ni * r) * inv;
         wi = (ni - nr * r) * inv;
     } else {
-        T r = dr / di, den = dr * r + di;
-        T inv = T(1) / den;
+        T r = dr * recip(di), den = dr * r + di;
+        T inv = recip(den);
         wr = (nr * r + ni) * inv;
         wi = (ni * r - nr) * inv;
     }
@@ -332,7 +336,7 @@ __device__ __forceinline__ void unno(const Geom<T> &g, const Samp<T> &s, UR<T> &
     r.gp3 = ei2 + auxq + auxu + auxv;
     r.gp2 = eq * rq + eu * ru + ev * rv;
     r.dt = ei2 * r.gp1 - r.gp2 * r.gp2;
-    r.dti = T(1) / r.dt;
+    r.dti = recip(r.dt);
     T gp4 = ei2 * eq + ei * (ev * ru - eu * rv);
     T gp5 = ei2 * eu + ei * (eq * rv - ev * rq);
     T gp6 = (ei2) * ev + ei * (eu * rq - eq * ru);

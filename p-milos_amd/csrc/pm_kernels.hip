@@ -32,11 +32,13 @@ template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM
 template <typename T, int W, int NLPL, int NT>
 struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
-    static constexpr int PAD_BYTES = 4 * PADW * (int)sizeof(T);
-    static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
-    // J^T J cross-lane reduction: CH values per chunk, double per lane
     static constexpr int CH = W >= 16 ? 16 : W;
     static constexpr int RED_BYTES = (CH + 1) * W * 8;   // lane stride CH+1: no bank conflicts
+    // rows per PSF pass: 8 (two parameters) when they fit in the space the
+    // J^T J reduction needs anyway, else 4
+    static constexpr int PAD_ROWS = 8 * PADW * (int)sizeof(T) <= RED_BYTES ? 8 : 4;
+    static constexpr int PAD_BYTES = PAD_ROWS * PADW * (int)sizeof(T);
+    static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
     static constexpr int U0 = PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES;
     // the PSF row buffer, the eigen fallback and the J^T J reduction share
     // one region (the row buffer is re-zeroed after the other two)
@@ -54,59 +56,66 @@ struct Lds {
 
 // ------------------------------------------------------------ convolution
 
-// Zero-padded "same" correlation of the four Stokes rows x[s][slot] with the
-// PSF, in place (convolution.c:54-78); when ic_row0 the I row uses the
-// continuum trick of convolution.c:90-113 with Ic = max(first, last) of the
-// row (mil_sinrf.c:390-395, me_der.c:478-483).  pad: 4 rows of PADW values,
-// zero outside [L/2, L/2 + L).  The FP64 form accumulates in double as
-// convolution.c:68 does; the mixed form in FP32 (kernel taps rounded once).
-template <typename T, int W, int NLPL>
-__device__ __forceinline__ void conv4(const PmConst *__restrict__ c, T *pad, int t, int L,
-                                      T x[4][NLPL], bool ic_row0)
+// Zero-padded "same" correlation of R rows x[r][slot] with the PSF, in place
+// (convolution.c:54-78).  Rows 0 and (when R == 8) 4 are Stokes I rows and
+// use the continuum trick of convolution.c:90-113 with Ic = max(first, last)
+// of that row (mil_sinrf.c:390-395, me_der.c:478-483), taken by in-group
+// shuffles so the row goes to LDS once.  pad: R rows of PADW values, zero
+// outside [L/2, L/2 + L).  The FP64 form accumulates in double as
+// convolution.c:68 does; the mixed form in FP32.
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int t, int L,
+                                      T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
     constexpr int PADW = 2 * W * NLPL;
     const int half = L / 2;
+    const int klast = (L - 1) / W, tlast = (L - 1) % W;
+    T ic[2];
+#pragma unroll
+    for (int q = 0; q < (R == 8 ? 2 : 1); q++) {
+        const int r = 4 * q;
+        T last = x[r][0];
+#pragma unroll
+        for (int k = 1; k < NLPL; k++) if (k == klast) last = x[r][k];
+        T a = __shfl(x[r][0], 0, W), b = __shfl(last, tlast, W);
+        ic[q] = (a > b) ? a : b;
+    }
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         if (lam < L) {
 #pragma unroll
-            for (int r = 0; r < 4; r++) pad[r * PADW + half + lam] = x[r][k];
+            for (int r = 0; r < R; r++) {
+                T v = x[r][k];
+                if (r == 0) v = ic[0] - v;
+                if (R == 8 && r == 4) v = ic[1] - v;
+                pad[r * PADW + half + lam] = v;
+            }
         }
     }
     wave_sync();
-    T ic = 0;
-    if (ic_row0) {
-        T a = pad[half], b = pad[half + L - 1];
-        ic = (a > b) ? a : b;
-        wave_sync();
-#pragma unroll
-        for (int k = 0; k < NLPL; k++) {
-            int lam = t + k * W;
-            if (lam < L) pad[half + lam] = ic - x[0][k];
-        }
-        wave_sync();
-    }
+    const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
+    const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         if (lam < L) {
-            A acc0 = 0, acc1 = 0, acc2 = 0, acc3 = 0;
+            A acc[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r] = 0;
             const T *p0 = pad + lam;
-            const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
-            const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
             for (int j = jlo; j < jhi; j++) {
                 A gj = A(c->G[j]);
-                acc0 += gj * (A)p0[j];
-                acc1 += gj * (A)p0[PADW + j];
-                acc2 += gj * (A)p0[2 * PADW + j];
-                acc3 += gj * (A)p0[3 * PADW + j];
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[r] += gj * (A)p0[r * PADW + j];
             }
-            x[0][k] = ic_row0 ? T((A)ic - acc0) : T(acc0);
-            x[1][k] = T(acc1);
-            x[2][k] = T(acc2);
-            x[3][k] = T(acc3);
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                if (r == 0) x[r][k] = T((A)ic[0] - acc[r]);
+                else if (R == 8 && r == 4) x[r][k] = T((A)ic[1] - acc[r]);
+                else x[r][k] = T(acc[r]);
+            }
         }
     }
     wave_sync();
@@ -141,7 +150,7 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
 #pragma unroll
         for (int s = 0; s < 4; s++) S[s][k] = s4[s];
     }
-    if (c->use_psf) conv4<T, W, NLPL>(c, pad, t, L, S, true);   // mil_sinrf.c:387-402
+    if (c->use_psf) convR<T, W, NLPL, 4>(c, pad, t, L, S);     // mil_sinrf.c:387-402
 }
 
 // Response functions of the synthesis held in sp, convolved like the
@@ -158,19 +167,33 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Ge
         response<T, NT>(c, g, sp[k], r, D[k]);
     }
     if (c->use_psf) {
+        // two parameters per pass where the row buffer holds 8 rows
+        constexpr int R = Lds<T, W, NLPL, NT>::PAD_ROWS;
+        constexpr int PPC = R / 4;
+        constexpr int NPS = NT - 1;                      // every row but S0's
 #pragma unroll
-        for (int p = 0; p < NT; p++) {
-            if (p == 7) continue;
-            T x[4][NLPL];
+        for (int q0 = 0; q0 < NPS; q0 += PPC) {
+            T x[R][NLPL];
 #pragma unroll
-            for (int k = 0; k < NLPL; k++)
+            for (int u = 0; u < PPC; u++) {
+                int q = q0 + u < NPS ? q0 + u : NPS - 1;
+                int p = q < 7 ? q : q + 1;
 #pragma unroll
-                for (int s = 0; s < 4; s++) x[s][k] = D[k][s][p];
-            conv4<T, W, NLPL>(c, pad, t, L, x, true);
+                for (int k = 0; k < NLPL; k++)
 #pragma unroll
-            for (int k = 0; k < NLPL; k++)
+                    for (int s = 0; s < 4; s++) x[4 * u + s][k] = D[k][s][p];
+            }
+            convR<T, W, NLPL, R>(c, pad, t, L, x);
 #pragma unroll
-                for (int s = 0; s < 4; s++) D[k][s][p] = x[s][k];
+            for (int u = 0; u < PPC; u++) {
+                if (q0 + u >= NPS) continue;
+                int q = q0 + u;
+                int p = q < 7 ? q : q + 1;
+#pragma unroll
+                for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                    for (int s = 0; s < 4; s++) D[k][s][p] = x[4 * u + s][k];
+            }
         }
     }
 }
@@ -240,7 +263,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
     // in LDS; lane 0 writes them
     double *lm = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES);
     double *red = (double *)pad;     // J^T J reduction (shares the row buffer)
-    for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+    for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
 
     const int xcd = blockIdx.x % PM_NSLICE;
@@ -338,7 +361,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
 #pragma unroll
                 for (int i = 0; i < NT; i++) delta[i] = ws[2 * NT * NT + NT + i];
                 wave_sync();
-                for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+                for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
                 wave_sync();
                 if (t == 0) atomicAdd(a.fallbacks, 1ull);
             }
@@ -473,7 +496,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
                 if (part == 0 && e < NV) ab[e] = sum;
                 wave_sync();
             }
-            for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+            for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
             wave_sync();
             wave_sync();
         }
@@ -511,7 +534,7 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
     const int L = c->L;
     const int grp = threadIdx.x / W, t = threadIdx.x % W;
     T *pad = (T *)(smem + grp * LD::GROUP_BYTES);
-    for (int i = t; i < 4 * LD::PADW; i += W) pad[i] = T(0);
+    for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
     const int ngroups = gridDim.x * LD::GPB;
     for (int pix = blockIdx.x * LD::GPB + grp; pix < n; pix += ngroups) {

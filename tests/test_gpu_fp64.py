@@ -90,7 +90,8 @@ def _compare_inversions(g, o, label, tol=TOL64, frac=0.995):
         "mean |d iters| %.3f" % (label, same_it, g["iters"].mean(), o["iters"].mean(),
                                  np.abs(g["iters"] - o["iters"]).mean()))
     assert np.all(ok.mean(axis=0) >= frac), ok.mean(axis=0)
-    both = np.all(ok, axis=1)
+    # chi2 agrees wherever the models agree to the tight FP64 tolerance
+    both = np.all(dm <= TOL64, axis=1)
     np.testing.assert_allclose(g["chi2"][both], o["chi2"][both], rtol=1e-4)
 
 
