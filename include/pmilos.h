@@ -67,6 +67,9 @@ typedef struct {
     int logclambda;              /* Diagonal element acceleration */
     double guess[PM_NPAR];       /* initial model (INITIAL_MODEL) */
     int precision;               /* enum pm_precision */
+    const float *slight;         /* [4][L] stray-light profile (the .per file of
+                                    readPerStrayLightFile, utilsFits.c:835-877),
+                                    or NULL */
 } pm_config;
 
 const char *pm_strerror(int status);
@@ -110,6 +113,16 @@ int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models,
 int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
                      float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
                      void *stream);
+
+/* Inversion with a stray-light profile per pixel: slight [npix][4][L] float
+ * (the 4-D stray-light cube of readFitsStrayLightFile, utilsFits.c:881,
+ * indexed per pixel as milos.c:986-996 does).  NULL = the context's profile
+ * (pm_config.slight) or none.  Otherwise as pm_invert / pm_invert_device. */
+int pm_invert_sl(pm_ctx *ctx, const float *stokes, const float *slight, int npix,
+                 double *models, float *chi2, int *iters, int *accepted, float *fit);
+int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
+                        double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
+                        float *d_fit, void *stream);
 
 /* Number of pixels whose 9x9/10x10/11x11 solve needed the eigen
  * pseudo-inverse fallback in the last inversion on this context (diagnostic;

@@ -42,6 +42,10 @@ struct PmConst {
     // PSF taps [glo, ghi) whose weight reaches 1e-17 (FP64 form) / 1e-9
     // (FP32 form) of the peak: the others are below the rounding of the sum
     int glo64, ghi64, glo32, ghi32;
+    // stray-light profile [4][L] mixed in with the filling factor
+    // (mil_sinrf.c:405-412), when use_slight
+    int use_slight;
+    float slight[4 * PM_LMAX];
 };
 
 // ------------------------------------------------------------ group helpers

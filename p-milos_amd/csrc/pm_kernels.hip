@@ -29,7 +29,7 @@ template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM
 
 // ------------------------------------------------------------ LDS layout
 
-template <typename T, int W, int NLPL, int NT>
+template <typename T, int W, int NLPL, int NT, bool GEN = false>
 struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
     static constexpr int CH = W >= 16 ? 16 : W;
@@ -47,7 +47,9 @@ struct Lds {
     static constexpr int NP = NT * (NT + 1) / 2;
     static constexpr int AB_BYTES = ((NP + NT) * 8 + 15) / 16 * 16;
     static constexpr int LM_BYTES = 24 * 8;                   // model[11], trial[11]
-    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES;
+    // macroturbulence kernel and its v_mac derivative, both PSF-folded
+    static constexpr int MAC_BYTES = GEN ? ((2 * W * NLPL * (int)sizeof(T) + 15) / 16 * 16) : 0;
+    static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES + MAC_BYTES;
     static constexpr int BLOCK = BlockOf<W>::value;
     static constexpr int GPB = BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
@@ -63,7 +65,7 @@ struct Lds {
 // shuffles so the row goes to LDS once.  pad: R rows of PADW values, zero
 // outside [L/2, L/2 + L).  The FP64 form accumulates in double as
 // convolution.c:68 does; the mixed form in FP32.
-template <typename T, int W, int NLPL, int R>
+template <typename T, int W, int NLPL, int R, bool IC = true>
 __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int t, int L,
                                       T x[R][NLPL])
 {
@@ -71,9 +73,9 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
     constexpr int PADW = 2 * W * NLPL;
     const int half = L / 2;
     const int klast = (L - 1) / W, tlast = (L - 1) % W;
-    T ic[2];
+    T ic[2] = {T(0), T(0)};
 #pragma unroll
-    for (int q = 0; q < (R == 8 ? 2 : 1); q++) {
+    for (int q = 0; q < (IC ? (R == 8 ? 2 : 1) : 0); q++) {
         const int r = 4 * q;
         T last = x[r][0];
 #pragma unroll
@@ -88,8 +90,8 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 T v = x[r][k];
-                if (r == 0) v = ic[0] - v;
-                if (R == 8 && r == 4) v = ic[1] - v;
+                if (IC && r == 0) v = ic[0] - v;
+                if (IC && R == 8 && r == 4) v = ic[1] - v;
                 pad[r * PADW + half + lam] = v;
             }
         }
@@ -112,14 +114,105 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
-                if (r == 0) x[r][k] = T((A)ic[0] - acc[r]);
-                else if (R == 8 && r == 4) x[r][k] = T((A)ic[1] - acc[r]);
+                if (IC && r == 0) x[r][k] = T((A)ic[0] - acc[r]);
+                else if (IC && R == 8 && r == 4) x[r][k] = T((A)ic[1] - acc[r]);
                 else x[r][k] = T(acc[r]);
             }
         }
     }
     wave_sync();
 }
+
+// Circular convolution with the kernel h (LDS, length L), rotated so that
+// out[m] = conv[(m + L/2) mod L] (convCircular, convolution.c:126-153); the
+// rows are staged in the middle of the PSF row buffer, leaving its zero
+// margins untouched.
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ void convC(T *pad, const T *h, int t, int L, T x[R][NLPL])
+{
+    typedef typename AccT<T>::type A;
+    constexpr int PADW = 2 * W * NLPL;
+    const int half = L / 2;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L)
+#pragma unroll
+            for (int r = 0; r < R; r++) pad[r * PADW + half + lam] = x[r][k];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int m = t + k * W;
+        if (m < L) {
+            int i = m + half;
+            if (i >= L) i -= L;
+            A acc[R];
+#pragma unroll
+            for (int r = 0; r < R; r++) acc[r] = 0;
+            for (int j = 0; j < L; j++) {
+                int idx = i - j;
+                if (idx < 0) idx += L;
+                A hj = (A)h[j];
+#pragma unroll
+                for (int r = 0; r < R; r++) acc[r] += hj * (A)pad[r * PADW + half + idx];
+            }
+#pragma unroll
+            for (int r = 0; r < R; r++) x[r][k] = T(acc[r]);
+        }
+    }
+    wave_sync();
+}
+
+// Macroturbulence kernel of model m and its v_mac derivative (fgauss,
+// fgauss.c:16-66; the derivative as me_der.c:328-329,410), both correlated
+// with the PSF when there is one (mil_sinrf.c:348-350, me_der.c:412-416);
+// written to gm / gd in LDS.
+template <typename T, int W, int NLPL>
+__device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const double *m,
+                                            T *pad, int t, int L, T *gm, T *gd)
+{
+    const double MC = m[9];
+    const double ild = (c->lambda0 * MC) / 2.99792458e5;
+    const double centro = c->lam[L / 2];
+    double gv[NLPL], a2[NLPL], part = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        gv[k] = 0;
+        a2[k] = 0;
+        if (lam < L) {
+            double a = (c->lam[lam] - centro) / ild;
+            double term = (a * a) / 2;
+            gv[k] = term < 1e30 ? exp(-term) : 0.0;
+            a2[k] = a * a;
+            part += gv[k];
+        }
+    }
+    const double sum = group_sum<W>(part);
+    T x[2][NLPL];
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        double g = gv[k] / sum;
+        x[0][k] = T(g);
+        x[1][k] = T(g / MC * (a2[k] - 1.0));
+    }
+    if (c->use_psf) convR<T, W, NLPL, 2, false>(c, pad, t, L, x);
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L) { gm[lam] = x[0][k]; gd[lam] = x[1][k]; }
+    }
+    wave_sync();
+}
+
+// What the general form keeps from a synthesis for its response functions.
+template <typename T, int NLPL>
+struct Extra {
+    T Sraw[4][NLPL];     // before any convolution (spectra_mac)
+    T Ssl[4][NLPL];      // after convolution, before the stray-light mix
+    bool mac;            // macroturbulence applied (v_mac > 1e-4)
+};
 
 // Zero the response-function row of a fixed parameter,
 // FijaACeroDerivadasNoNecesarias, milosUtils.c:199-227.
@@ -131,11 +224,14 @@ __device__ __forceinline__ bool row_is_fixed(const PmConst *__restrict__ c, int 
 }
 
 // Synthesis of model m for this lane's wavelength slots: fills the sample
-// state (for the RFs) and the emergent, convolved Stokes vector S.
-template <typename T, int W, int NLPL>
+// state (for the RFs) and the emergent Stokes vector S, convolved
+// (mil_sinrf.c:287-403) and mixed with the stray light (:405-412).  The
+// general form (GEN) also handles macroturbulence and stray light.
+template <typename T, int W, int NLPL, bool GEN>
 __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const double *m,
-                                            T *pad, int t, int L, Geom<T> &g,
-                                            Samp<T> sp[NLPL], T S[4][NLPL])
+                                            const float *slp, T *pad, T *gm, T *gd, int t,
+                                            int L, Geom<T> &g, Samp<T> sp[NLPL], T S[4][NLPL],
+                                            Extra<T, NLPL> &ex)
 {
     make_geom<T>(c, m, g);
 #pragma unroll
@@ -150,15 +246,49 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
 #pragma unroll
         for (int s = 0; s < 4; s++) S[s][k] = s4[s];
     }
-    if (c->use_psf) convR<T, W, NLPL, 4>(c, pad, t, L, S);     // mil_sinrf.c:387-402
+    ex.mac = false;
+    if (GEN) {
+#pragma unroll
+        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) ex.Sraw[s][k] = S[s][k];
+        ex.mac = m[9] > 0.0001;                                    // mil_sinrf.c:288
+    }
+    if (ex.mac) {
+        mac_kernels<T, W, NLPL>(c, m, pad, t, L, gm, gd);
+        convC<T, W, NLPL, 4>(pad, gm, t, L, S);                    // :353-354
+    } else if (c->use_psf) {
+        convR<T, W, NLPL, 4>(c, pad, t, L, S);                     // :387-402
+    }
+    if (GEN && slp) {                                              // :405-412
+        const T alf = T(m[10]);
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            int lam = t + k * W;
+            int li = lam < L ? lam : L - 1;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                ex.Ssl[s][k] = S[s][k];
+                S[s][k] = S[s][k] * alf + T(slp[s * L + li]) * (T(1) - alf);
+            }
+        }
+    }
 }
 
 // Response functions of the synthesis held in sp, convolved like the
-// spectra (me_der.c:471-495, S0 row never), one parameter at a time.
-template <typename T, int W, int NLPL, int NT>
-__device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Geom<T> &g,
-                                         const Samp<T> sp[NLPL], T *pad, int t, int L,
-                                         T D[NLPL][4][NT])
+// spectra (me_der.c:316-497, the S0 row never), two parameters per pass
+// where the row buffer holds 8 rows.  The general form adds the
+// macroturbulence row (row 9, me_der.c:418-425) and the stray-light scaling
+// and filling-factor row (me_der.c:499-525); ff_src is the reference's
+// "spectra_slight" argument of that call: the synthesis before the mix on
+// the first call (milosUtils.c:537), the mixed one after an accepted step
+// (:587), selected by ff_pre.
+template <typename T, int W, int NLPL, int NT, bool GEN>
+__device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const double *m,
+                                         const float *slp, const Geom<T> &g,
+                                         const Samp<T> sp[NLPL], T *pad, const T *gm,
+                                         const T *gd, int t, int L, const Extra<T, NLPL> &ex,
+                                         bool ff_pre, const T S[4][NLPL], T D[NLPL][4][NT])
 {
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
@@ -166,11 +296,23 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Ge
         unno<T>(g, sp[k], r);
         response<T, NT>(c, g, sp[k], r, D[k]);
     }
-    if (c->use_psf) {
-        // two parameters per pass where the row buffer holds 8 rows
-        constexpr int R = Lds<T, W, NLPL, NT>::PAD_ROWS;
-        constexpr int PPC = R / 4;
-        constexpr int NPS = NT - 1;                      // every row but S0's
+    constexpr int R = Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
+    constexpr int PPC = R / 4;
+    constexpr int NPS = 8;                      // rows 0..8 but S0's
+    const bool mac = GEN && ex.mac;
+    if (mac && NT >= 10) {                      // v_mac row from the raw synthesis
+        T x[4][NLPL];
+#pragma unroll
+        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) x[s][k] = ex.Sraw[s][k];
+        convC<T, W, NLPL, 4>(pad, gd, t, L, x);
+#pragma unroll
+        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) D[k][s][NT >= 10 ? 9 : 0] = x[s][k];
+    }
+    if (mac || c->use_psf) {
 #pragma unroll
         for (int q0 = 0; q0 < NPS; q0 += PPC) {
             T x[R][NLPL];
@@ -183,7 +325,8 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Ge
 #pragma unroll
                     for (int s = 0; s < 4; s++) x[4 * u + s][k] = D[k][s][p];
             }
-            convR<T, W, NLPL, R>(c, pad, t, L, x);
+            if (mac) convC<T, W, NLPL, R>(pad, gm, t, L, x);
+            else convR<T, W, NLPL, R>(c, pad, t, L, x);
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
                 if (q0 + u >= NPS) continue;
@@ -194,6 +337,22 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const Ge
 #pragma unroll
                     for (int s = 0; s < 4; s++) D[k][s][p] = x[4 * u + s][k];
             }
+        }
+    }
+    if (GEN && slp) {
+        const T alf = T(m[10]);
+        const int ff = NT == 11 ? 10 : (c->fix[9] ? 10 : 9);
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            int lam = t + k * W;
+            int li = lam < L ? lam : L - 1;
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int p = 0; p < NT; p++) {
+                    D[k][s][p] = D[k][s][p] * alf;
+                    if (p == ff) D[k][s][p] = (ff_pre ? ex.Ssl[s][k] : S[s][k]) - T(slp[s * L + li]);
+                }
         }
     }
 }
@@ -212,6 +371,7 @@ struct InvArgs {
     float *fit;
     int *counters;
     unsigned long long *fallbacks;
+    const float *slight;     // per-pixel stray light [npix][4][L], or null
 };
 
 __device__ __forceinline__ void fetch_chunk(int *counters, int xcd, int npix, int slice,
@@ -246,10 +406,18 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 
 // One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
 // loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
-template <typename T, int W, int NLPL, int NT>
-__global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_invert_kernel(InvArgs a)
+//
+// GEN = false: NTERMS 9 with neither macroturbulence nor stray light.
+// GEN = true (NT = 11): the reference's NTERMS 9, 10 or 11 in the 11-row
+// layout (v_mac in row 9, the filling factor in row 10); rows the
+// reference's run does not have are fixed rows here, zeroed like FijaACero
+// does, so alpha, beta and delta agree with the reference's NTERMS-sized
+// ones on the rows it has.  nfree uses the reference's NTERMS.
+template <typename T, int W, int NLPL, int NT, bool GEN>
+__global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 && !GEN ? 2 : 1))
+pm_invert_kernel(InvArgs a)
 {
-    using LD = Lds<T, W, NLPL, NT>;
+    using LD = Lds<T, W, NLPL, NT, GEN>;
     constexpr int NP = LD::NP;
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
     const PmConst *__restrict__ c = a.c;
@@ -263,6 +431,8 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
     // in LDS; lane 0 writes them
     double *lm = (double *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES);
     double *red = (double *)pad;     // J^T J reduction (shares the row buffer)
+    T *gm = (T *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
+    T *gd = gm + W * NLPL;
     for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
 
@@ -276,6 +446,8 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
     T S[4][NLPL];
     Samp<T> sp[NLPL];
     Geom<T> g;
+    Extra<T, NLPL> ex;
+    const float *slp = nullptr;
 
     while (true) {
         bool first = false;
@@ -292,6 +464,9 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
             }
             pix = cur++;
             const float *src = a.stokes + (size_t)pix * 4 * L;
+            if (GEN)                                          // milos.c:986-996
+                slp = a.slight ? a.slight + (size_t)pix * 4 * L
+                               : (c->use_slight ? c->slight : nullptr);
             int ng = 0;
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
@@ -307,7 +482,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
                 }
             }
             wave_sync();
-            nfree = 4 * L - NT - group_sum_int<W>(ng);        // :507-508
+            nfree = 4 * L - c->NT - group_sum_int<W>(ng);     // :507-508
             {
                 Classic e = classical(c, sobs);
                 wave_sync();
@@ -379,7 +554,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
         }
 
         // trial synthesis and chi2, :569-571 (fchisqr, lib.c:126-152)
-        synth_slots<T, W, NLPL>(c, lm + 11, pad, t, L, g, sp, S);
+        synth_slots<T, W, NLPL, GEN>(c, lm + 11, slp, pad, gm, gd, t, L, g, sp, S, ex);
         double part = 0;
 #pragma unroll
         for (int k = 0; k < NLPL; k++) {
@@ -427,7 +602,7 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
 
         if (do_rf) {
             T D[NLPL][4][NT];
-            rf_slots<T, W, NLPL, NT>(c, g, sp, pad, t, L, D);
+            rf_slots<T, W, NLPL, NT, GEN>(c, lm, slp, g, sp, pad, gm, gd, t, L, ex, first, S, D);
             // covarm2 (lib.c:71-116): each alpha/beta entry is a per-lane
             // partial sum followed by the group sum; lane 0 stores it in LDS
 #pragma unroll
@@ -525,15 +700,24 @@ __global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 ? 2 : 1)) pm_inv
 
 // ------------------------------------------------------------ synthesis kernel
 
-template <typename T, int W, int NLPL, int NT>
+// RF rows are written in the reference's NTERMS layout (me_der.c); the
+// general form maps its 11-row layout back (row 9 of an NTERMS-10 run is
+// the filling factor when me_der puts it there, me_der.c:505-520).
+template <typename T, int W, int NLPL, int NT, bool GEN>
 __global__ void __launch_bounds__(BlockOf<W>::value)
 pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, double *rf)
 {
-    using LD = Lds<T, W, NLPL, NT>;
+    using LD = Lds<T, W, NLPL, NT, GEN>;
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
     const int L = c->L;
     const int grp = threadIdx.x / W, t = threadIdx.x % W;
-    T *pad = (T *)(smem + grp * LD::GROUP_BYTES);
+    char *gb = smem + grp * LD::GROUP_BYTES;
+    T *pad = (T *)gb;
+    T *gm = (T *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
+    T *gd = gm + W * NLPL;
+    const float *slp = (GEN && c->use_slight) ? c->slight : nullptr;
+    const int NTR = c->NT;
+    const int row9 = (NTR == 10 && slp && !c->fix[9]) ? 10 : 9;
     for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
     wave_sync();
     const int ngroups = gridDim.x * LD::GPB;
@@ -544,7 +728,8 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
         Geom<T> g;
         Samp<T> sp[NLPL];
         T S[4][NLPL];
-        synth_slots<T, W, NLPL>(c, m, pad, t, L, g, sp, S);
+        Extra<T, NLPL> ex;
+        synth_slots<T, W, NLPL, GEN>(c, m, slp, pad, gm, gd, t, L, g, sp, S, ex);
 #pragma unroll
         for (int k = 0; k < NLPL; k++) {
             int lam = t + k * W;
@@ -555,7 +740,7 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
         }
         if (rf) {
             T D[NLPL][4][NT];
-            rf_slots<T, W, NLPL, NT>(c, g, sp, pad, t, L, D);
+            rf_slots<T, W, NLPL, NT, GEN>(c, m, slp, g, sp, pad, gm, gd, t, L, ex, true, S, D);
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
                 int lam = t + k * W;
@@ -563,9 +748,18 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
 #pragma unroll
                     for (int s = 0; s < 4; s++)
 #pragma unroll
-                        for (int p = 0; p < NT; p++)
-                            rf[(size_t)pix * 4 * NT * L + (size_t)s * NT * L + p * L + lam] =
+                        for (int p = 0; p < NT; p++) {
+                            int q = p;                       // row in the reference layout
+                            if (GEN && p >= 9) {
+                                if (NTR == 9) continue;
+                                if (NTR == 10) {
+                                    if (p != row9) continue;
+                                    q = 9;
+                                }
+                            }
+                            rf[(size_t)pix * 4 * NTR * L + (size_t)s * NTR * L + q * L + lam] =
                                 (double)D[k][s][p];
+                        }
             }
         }
     }
@@ -717,8 +911,10 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     if (rc != PM_OK) { delete ctx; return rc; }
     c.L = L;
     c.NT = pm_nterms(cfg->fix, cfg->guess[9]);
-    // macroturbulence and stray light are not in these kernels yet
-    if (c.NT != 9 || cfg->guess[9] > 0.0001) { delete ctx; return PM_ENOTSUP; }
+    if (cfg->slight) {
+        c.use_slight = 1;
+        for (int i = 0; i < 4 * L; i++) c.slight[i] = cfg->slight[i];
+    }
     c.use_psf = cfg->psf ? 1 : 0;
     c.miter = cfg->miter;
     c.logclambda = cfg->logclambda;
@@ -780,11 +976,11 @@ extern "C" int pm_ctx_nterms(const pm_ctx *ctx) { return ctx ? ctx->h.NT : PM_EI
 
 // ------------------------------------------------------------ dispatch
 
-template <typename T, int W, int NLPL, int NT>
+template <typename T, int W, int NLPL, int NT, bool GEN>
 static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
 {
-    using LD = Lds<T, W, NLPL, NT>;
-    auto kern = pm_invert_kernel<T, W, NLPL, NT>;
+    using LD = Lds<T, W, NLPL, NT, GEN>;
+    auto kern = pm_invert_kernel<T, W, NLPL, NT, GEN>;
     int per_cu = 0;
     PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LD::BLOCK, 0));
     if (per_cu < 1) per_cu = 1;
@@ -799,16 +995,16 @@ static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
     return PM_OK;
 }
 
-template <typename T, int W, int NLPL, int NT>
+template <typename T, int W, int NLPL, int NT, bool GEN>
 static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double *rf,
                         hipStream_t st)
 {
-    using LD = Lds<T, W, NLPL, NT>;
+    using LD = Lds<T, W, NLPL, NT, GEN>;
     long long need = ((long long)n + LD::GPB - 1) / LD::GPB;
     long long grid = (long long)ctx->ncu * 8;
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL((pm_synth_kernel<T, W, NLPL, NT>), dim3((unsigned)grid), dim3(LD::BLOCK),
+    hipLaunchKernelGGL((pm_synth_kernel<T, W, NLPL, NT, GEN>), dim3((unsigned)grid), dim3(LD::BLOCK),
                        0, st, (const PmConst *)ctx->d, m, n, sp, rf);
     PM_HIP(hipGetLastError());
     return PM_OK;
@@ -816,16 +1012,28 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
 
 // Lanes per pixel for L wavelengths: the smallest power of two >= L up to a
 // full wavefront, then two samples per lane.  T = double (FP64 form) or float
-// (mixed form: spectra, RFs and convolution in FP32).
-#define PM_DISPATCH_LT(T, L, FN, ...)                                         \
-    ((L) <= 8 ? FN<T, 8, 1, 9>(__VA_ARGS__) :                                 \
-     (L) <= 16 ? FN<T, 16, 1, 9>(__VA_ARGS__) :                               \
-     (L) <= 32 ? FN<T, 32, 1, 9>(__VA_ARGS__) :                               \
-     (L) <= 64 ? FN<T, 64, 1, 9>(__VA_ARGS__) :                               \
-                 FN<T, 64, 2, 9>(__VA_ARGS__))
-#define PM_DISPATCH_L(ctx, L, FN, ...)                                        \
-    ((ctx)->precision == PM_MIXED ? PM_DISPATCH_LT(float, L, FN, __VA_ARGS__) \
-                                  : PM_DISPATCH_LT(double, L, FN, __VA_ARGS__))
+// (mixed form: spectra, RFs and convolution in FP32).  The general form
+// (NTERMS 10/11, macroturbulence, stray light) is the 11-row kernel.
+#define PM_DISPATCH_LT(T, NT, G, L, FN, ...)                                  \
+    ((L) <= 8 ? FN<T, 8, 1, NT, G>(__VA_ARGS__) :                             \
+     (L) <= 16 ? FN<T, 16, 1, NT, G>(__VA_ARGS__) :                           \
+     (L) <= 32 ? FN<T, 32, 1, NT, G>(__VA_ARGS__) :                           \
+     (L) <= 64 ? FN<T, 64, 1, NT, G>(__VA_ARGS__) :                           \
+                 FN<T, 64, 2, NT, G>(__VA_ARGS__))
+#define PM_DISPATCH_L(ctx, gen, L, FN, ...)                                   \
+    ((gen) ? ((ctx)->precision == PM_MIXED                                    \
+                  ? PM_DISPATCH_LT(float, 11, true, L, FN, __VA_ARGS__)       \
+                  : PM_DISPATCH_LT(double, 11, true, L, FN, __VA_ARGS__))     \
+           : ((ctx)->precision == PM_MIXED                                    \
+                  ? PM_DISPATCH_LT(float, 9, false, L, FN, __VA_ARGS__)       \
+                  : PM_DISPATCH_LT(double, 9, false, L, FN, __VA_ARGS__)))
+
+// The 9-row kernel covers NTERMS 9 without stray light; everything else
+// takes the general one.
+static bool needs_general(const pm_ctx *ctx, bool slight_cube)
+{
+    return ctx->h.NT != 9 || ctx->h.use_slight || slight_cube;
+}
 
 extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectra,
                                double *d_rf, void *stream)
@@ -835,12 +1043,13 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     hipStream_t st = (hipStream_t)stream;
     ctx->last_stream = st;
     const int L = ctx->h.L;
-    return PM_DISPATCH_L(ctx, L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
+    const bool gen = needs_general(ctx, false);
+    return PM_DISPATCH_L(ctx, gen, L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
 }
 
-extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
-                                float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
-                                void *stream)
+extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_slight,
+                                   int npix, double *d_models, float *d_chi2, int *d_iters,
+                                   int *d_accepted, float *d_fit, void *stream)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
@@ -859,8 +1068,18 @@ extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, do
     a.fit = d_fit;
     a.counters = ctx->counters;
     a.fallbacks = ctx->fallbacks;
+    a.slight = d_slight;
     const int L = ctx->h.L;
-    return PM_DISPATCH_L(ctx, L, launch_invert, ctx, a, st);
+    const bool gen = needs_general(ctx, d_slight != nullptr);
+    return PM_DISPATCH_L(ctx, gen, L, launch_invert, ctx, a, st);
+}
+
+extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
+                                float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
+                                void *stream)
+{
+    return pm_invert_sl_device(ctx, d_stokes, nullptr, npix, d_models, d_chi2, d_iters,
+                               d_accepted, d_fit, stream);
 }
 
 extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
@@ -902,22 +1121,25 @@ extern "C" int pm_synth(pm_ctx *ctx, const double *models, int n, double *spectr
     return PM_OK;
 }
 
-extern "C" int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models, float *chi2,
-                         int *iters, int *accepted, float *fit)
+extern "C" int pm_invert_sl(pm_ctx *ctx, const float *stokes, const float *slight, int npix,
+                            double *models, float *chi2, int *iters, int *accepted, float *fit)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
     if (!stokes || !models || !chi2 || !iters) return PM_EINVAL;
     const size_t L = ctx->h.L, n = npix;
-    DevBuf ds, dm, dc, di, da, df;
+    DevBuf ds, dl, dm, dc, di, da, df;
     if (ds.alloc(sizeof(float) * 4 * L * n) || dm.alloc(sizeof(double) * 11 * n) ||
         dc.alloc(sizeof(float) * n) || di.alloc(sizeof(int) * n) ||
+        (slight && dl.alloc(sizeof(float) * 4 * L * n)) ||
         (accepted && da.alloc(sizeof(int) * n)) || (fit && df.alloc(sizeof(float) * 4 * L * n)))
         return PM_ENOMEM;
     PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
-    int rc = pm_invert_device(ctx, (const float *)ds.p, npix, (double *)dm.p, (float *)dc.p,
-                              (int *)di.p, accepted ? (int *)da.p : nullptr,
-                              fit ? (float *)df.p : nullptr, nullptr);
+    if (slight) PM_HIP(hipMemcpy(dl.p, slight, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
+    int rc = pm_invert_sl_device(ctx, (const float *)ds.p, slight ? (const float *)dl.p : nullptr,
+                                 npix, (double *)dm.p, (float *)dc.p, (int *)di.p,
+                                 accepted ? (int *)da.p : nullptr, fit ? (float *)df.p : nullptr,
+                                 nullptr);
     if (rc) return rc;
     PM_HIP(hipDeviceSynchronize());
     PM_HIP(hipMemcpy(models, dm.p, sizeof(double) * 11 * n, hipMemcpyDeviceToHost));
@@ -926,4 +1148,10 @@ extern "C" int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *mod
     if (accepted) PM_HIP(hipMemcpy(accepted, da.p, sizeof(int) * n, hipMemcpyDeviceToHost));
     if (fit) PM_HIP(hipMemcpy(fit, df.p, sizeof(float) * 4 * L * n, hipMemcpyDeviceToHost));
     return PM_OK;
+}
+
+extern "C" int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models, float *chi2,
+                         int *iters, int *accepted, float *fit)
+{
+    return pm_invert_sl(ctx, stokes, nullptr, npix, models, chi2, iters, accepted, fit);
 }

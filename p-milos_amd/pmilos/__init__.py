@@ -35,7 +35,8 @@ class Config(C.Structure):
                 ("fix", C.c_int * NPAR), ("weight", C.c_double * 4),
                 ("noise", C.c_double), ("ilambda", C.c_double), ("toplim", C.c_double),
                 ("miter", C.c_int), ("mu", C.c_double), ("logclambda", C.c_int),
-                ("guess", C.c_double * NPAR), ("precision", C.c_int)]
+                ("guess", C.c_double * NPAR), ("precision", C.c_int),
+                ("slight", C.POINTER(C.c_float))]
 
 
 def lib():
@@ -59,6 +60,8 @@ def lib():
         L.pm_synth_device.argtypes = [vp, vp, C.c_int, vp, vp, vp]
         L.pm_invert.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
         L.pm_invert_device.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.pm_invert_sl.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp]
+        L.pm_invert_sl_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.pm_last_fallback_count.argtypes = [vp, C.POINTER(C.c_longlong)]
         _lib = L
     return _lib
@@ -93,8 +96,10 @@ class Context:
 
     def __init__(self, lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
                  noise=1e-6, ilambda=float(np.float32(0.1)), toplim=1e-12, miter=50,
-                 mu=1.0, logclambda=0, guess=None, precision=PM_FP64):
+                 mu=1.0, logclambda=0, guess=None, precision=PM_FP64, slight=None):
         self._lam = np.ascontiguousarray(lam, dtype=np.float64)
+        self._slight = None if slight is None else \
+            np.ascontiguousarray(slight, dtype=np.float32).reshape(4, len(self._lam))
         self._psf = None if psf is None else np.ascontiguousarray(psf, dtype=np.float64)
         cfg = Config()
         cfg.nlambda = len(self._lam)
@@ -114,6 +119,8 @@ class Context:
         for i in range(NPAR):
             cfg.guess[i] = float(guess[i])
         cfg.precision = precision
+        if self._slight is not None:
+            cfg.slight = self._slight.ctypes.data_as(C.POINTER(C.c_float))
         h = C.c_void_p()
         _check(lib().pm_ctx_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -139,22 +146,33 @@ class Context:
         _check(lib().pm_synth(self.h, m.ctypes.data, n, spec.ctypes.data, _ptr(d)))
         return (spec, d) if rf else spec
 
-    def invert(self, stokes, want_fit=False):
+    def invert(self, stokes, want_fit=False, slight=None):
+        """slight: optional per-pixel stray light [n][4][L] (else the
+        context's profile, if any)."""
         st = np.ascontiguousarray(stokes, dtype=np.float32)
         n = st.shape[0]
+        sl = None if slight is None else np.ascontiguousarray(slight, dtype=np.float32)
+        if sl is not None and sl.shape != st.shape:
+            raise PmError("slight must have the shape of stokes")
         models = np.zeros((n, NPAR))
         chi2 = np.zeros(n, dtype=np.float32)
         iters = np.zeros(n, dtype=np.int32)
         acc = np.zeros(n, dtype=np.int32)
         fit = np.zeros((n, 4, self.L), dtype=np.float32) if want_fit else None
-        _check(lib().pm_invert(self.h, st.ctypes.data, n, models.ctypes.data,
-                               chi2.ctypes.data, iters.ctypes.data, acc.ctypes.data, _ptr(fit)))
+        _check(lib().pm_invert_sl(self.h, st.ctypes.data, _ptr(sl), n, models.ctypes.data,
+                                  chi2.ctypes.data, iters.ctypes.data, acc.ctypes.data,
+                                  _ptr(fit)))
         return dict(model=models, chi2=chi2, iters=iters, accepted=acc, fit=fit)
 
     def invert_ptrs(self, d_stokes, npix, d_models, d_chi2, d_iters, d_accepted=0, d_fit=0,
-                    stream=0):
-        _check(lib().pm_invert_device(self.h, d_stokes, npix, d_models, d_chi2, d_iters,
-                                      d_accepted or None, d_fit or None, stream or None))
+                    stream=0, d_slight=0):
+        if d_slight:
+            _check(lib().pm_invert_sl_device(self.h, d_stokes, d_slight, npix, d_models, d_chi2,
+                                             d_iters, d_accepted or None, d_fit or None,
+                                             stream or None))
+        else:
+            _check(lib().pm_invert_device(self.h, d_stokes, npix, d_models, d_chi2, d_iters,
+                                          d_accepted or None, d_fit or None, stream or None))
 
     def synth_ptrs(self, d_models, n, d_spectra, d_rf=0, stream=0):
         _check(lib().pm_synth_device(self.h, d_models, n, d_spectra, d_rf or None,

@@ -1,0 +1,240 @@
+"""GPU parity of the general form -- NTERMS 10 and 11, macroturbulence,
+stray light with its filling factor -- against the CPU oracle on the same
+seeded inputs, through the C ABI.  The reference's own golden synthesis
+(test/synthesis.mtrol, stray light with alpha 0.85) runs through the GPU."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle as O
+import pmilos
+from fixtures import (FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models,
+                      oracle_cube, read_fits_2d)
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GRIDS = {30: (FE6173, -290, 20, 290), 49: (FE6173, -600, 25, 600),
+         112: (FE6302, -1193.25, 21.5, 1193.25)}
+
+# (fix, guess v_mac, stray light) of each NTERMS layout the reference has
+# (milos.c:154-157; me_der.c:418-425 and :499-525 for where rows 9/10 go)
+CASES = {
+    "mac_nt10": ([1] * 10 + [0], 0.5, False),       # v_mac free, row 9
+    "mac_nt11_sl": ([1] * 11, 0.5, True),           # v_mac row 9, filling factor row 10
+    "ff_nt10_sl": ([1] * 9 + [0, 1], 0.0, True),    # no v_mac, filling factor row 9
+    "macfix_nt10_sl": ([1] * 9 + [0, 0], 0.5, True),  # v_mac fixed but applied; row 9
+    "nt9_sl": ([1] * 9 + [0, 0], 0.0, True),        # alpha fixed, stray light mixed in
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _slight_profile(lam, l0):
+    """A float32 stray-light profile: a broadened, weak-field line of the
+    same grid (what a quiet-Sun average looks like)."""
+    s = O.make_setup(l0, TRIPLET, lam)
+    m = np.array([6.0, 150.0, 0.3, 0.045, 0.2, 60.0, 20.0, 0.25, 0.75, 0, 1])
+    return O.synth(s, m).astype(np.float32)
+
+
+def _pair(case, L, psf, precision=pmilos.PM_FP64, **kw):
+    fix, gmac, use_sl = CASES[case]
+    g = GRIDS[L]
+    lam = O.grid(*g)
+    G = O.gauss_psf(60.0, lam) if psf else None
+    sl = _slight_profile(lam, g[0]) if use_sl else None
+    s = O.make_setup(g[0], TRIPLET, lam, psf=G, fix=fix, guess_mac=gmac,
+                     slight=None if sl is None else sl.astype(np.float64), **kw)
+    guess = GUESS.copy()
+    guess[9] = gmac
+    guess[10] = 1.0 if not use_sl else 0.9
+    ctx = pmilos.Context(g[0], TRIPLET, lam, psf=G, fix=fix, guess=guess, slight=sl,
+                         precision=precision, **kw)
+    assert ctx.NT == s.nterms
+    return s, ctx, guess, sl
+
+
+def _models(n, seed, case):
+    fix, gmac, use_sl = CASES[case]
+    m = draw_models(n, seed, mac_range=True)
+    rng = np.random.default_rng(seed + 7)
+    if use_sl:
+        m[:, 10] = rng.uniform(0.6, 1.0, n)
+    m[1, 9] = 0.0           # no macroturbulence: the PSF (or nothing) instead
+    m[2, 9] = 5e-5          # below the 1e-4 switch of mil_sinrf.c:288
+    m[3, 1] = 0.0           # B = 0
+    if not fix[9] and gmac == 0.0:
+        # NTERMS has no v_mac row: the reference never has v_mac > 0 here
+        m[:, 9] = 0.0
+    return m
+
+
+def _err_spectra(a, b):
+    floor = 1e-6 * np.abs(b[0]).max()
+    scale = np.maximum(np.abs(b).max(axis=-1, keepdims=True), floor)
+    return (np.abs(a - b) / scale).max()
+
+
+def _err_rf(a, b):
+    scale = np.maximum(np.abs(b).max(axis=(0, 2), keepdims=True), 1e-5)
+    return (np.abs(a - b) / scale).max()
+
+
+def _golden():
+    slight = read_fits_2d(os.path.join(GOLDEN, "straylight.fits"))
+    lam = O.grid(FE6173, -600, 25, 600)
+    keys = {"eta0": 0, "magnetic field": 1, "LOS velocity": 2, "Doppler width": 3,
+            "Damping": 4, "gamma": 5, "phi": 6, "S_0": 7, "S_1": 8, "v_mac": 9,
+            "filling factor": 10}
+    model = np.zeros(11)
+    for line in open(os.path.join(GOLDEN, "obs.mod")):
+        if ":" not in line:
+            continue
+        name, val = line.split(":", 1)
+        for k, i in keys.items():
+            if k in name:
+                model[i] = float(val.split()[0])
+    gold = np.loadtxt(os.path.join(GOLDEN, "synthesis_results.per"))[:, 2:].T
+    return slight, lam, model, gold
+
+
+@pytest.mark.parametrize("precision", [pmilos.PM_FP64, pmilos.PM_MIXED])
+def test_golden_synthesis_gpu(precision):
+    """test/synthesis.mtrol -> test/synthesis_results.per: 49 wavelengths,
+    no PSF, stray light mixed with filling factor 0.85, NTERMS 10."""
+    say("case golden synthesis precision=%d" % precision)
+    slight, lam, model, gold = _golden()
+    fix = [1] * 9 + [0, 1]
+    ctx = pmilos.Context(FE6173, TRIPLET, lam, fix=fix, guess=model, slight=slight,
+                         precision=precision)
+    s = O.make_setup(FE6173, TRIPLET, lam, fix=fix, guess_mac=model[9], slight=slight)
+    assert ctx.NT == s.nterms == 10
+    spec, rf = ctx.synth(model, rf=True)
+    spec, rf = spec[0], rf[0]
+    # the golden file is printed with %e (7 digits) from the float build
+    assert np.all(np.abs(spec - gold) <= 3e-7 + 1e-6 * np.abs(gold))
+    sp_o, rf_o = O.synth(s, model, rf=True)
+    if precision == pmilos.PM_FP64:
+        assert _err_spectra(spec, sp_o) < 1e-10
+        assert _err_rf(rf, rf_o) < 1e-10
+    else:
+        assert np.abs(spec - sp_o).max() < 1e-6 * np.abs(sp_o[0]).max()
+        assert _err_rf(rf, rf_o) < 2e-3
+
+
+@pytest.mark.parametrize("psf", [True, False])
+@pytest.mark.parametrize("case,L", [("mac_nt10", 112), ("mac_nt10", 30),
+                                    ("mac_nt11_sl", 49), ("mac_nt11_sl", 112),
+                                    ("ff_nt10_sl", 30), ("macfix_nt10_sl", 49),
+                                    ("nt9_sl", 30)])
+def test_synthesis_and_rf_general(case, L, psf):
+    """Spectra and response functions, rows in the reference's NTERMS layout
+    (the filling-factor row included), FP64 within 1e-10 and the mixed form
+    within its stated tolerances."""
+    say("case general synth %s L=%d psf=%s" % (case, L, psf))
+    s, ctx, guess, sl = _pair(case, L, psf)
+    _, ctxm, _, _ = _pair(case, L, psf, precision=pmilos.PM_MIXED)
+    models = _models(40, 2000 + L, case)
+    spec_g, rf_g = ctx.synth(models, rf=True)
+    spec_m, rf_m = ctxm.synth(models, rf=True)
+    for i, m in enumerate(models):
+        sp, rf = O.synth(s, m, rf=True)
+        e1, e2 = _err_spectra(spec_g[i], sp), _err_rf(rf_g[i], rf)
+        assert e1 < 1e-10, (i, e1)
+        assert e2 < 1e-10, (i, e2)
+        em = np.abs(spec_m[i] - sp).max() / np.abs(sp[0]).max()
+        assert em < 1e-6, (i, em)
+        erm = _err_rf(rf_m[i], rf)
+        assert erm < 2e-3, (i, erm)
+
+
+# per-parameter tolerance of the general FP64 inversion against the oracle
+# (absolute, Init_Model order incl. v_mac [km/s] and the filling factor).
+# Wide grids with a free v_mac give the LM several nearly flat directions;
+# rounding-level differences (summation order, LDL^T vs eigen solve) part
+# a few paths, as with the 112-sample grid of the 9-parameter form.
+TOL_GEN = np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 5e-3, 5e-3, 5e-3, 5e-3])
+
+
+def _compare(g, o, label, tol, frac):
+    dm = np.abs(g["model"] - o["model"])
+    ok = dm <= tol
+    say("%s: fraction within tolerance %s" % (label, np.round(ok.mean(axis=0), 4)))
+    say("%s: 99th percentile |d| %s" % (label, np.array2string(
+        np.percentile(dm, 99, axis=0), precision=3)))
+    say("%s: iterations identical on %.4f; mean iters gpu %.2f oracle %.2f"
+        % (label, (g["iters"] == o["iters"]).mean(), g["iters"].mean(), o["iters"].mean()))
+    assert np.all(ok.mean(axis=0) >= frac), ok.mean(axis=0)
+
+
+@pytest.mark.parametrize("case,L,psf", [("mac_nt10", 112, True), ("mac_nt11_sl", 49, False),
+                                        ("ff_nt10_sl", 30, True), ("nt9_sl", 30, True)])
+def test_invert_general_fp64(case, L, psf):
+    """Inversions of a seeded cube (sigma 1e-3 Ic) with v_mac ~ U(0, 2) km/s
+    and, with stray light, alpha ~ U(0.6, 1): FP64 against the oracle."""
+    n = 256
+    say("case general invert %s L=%d psf=%s npix=%d" % (case, L, psf, n))
+    s, ctx, guess, sl = _pair(case, L, psf)
+    truths = _models(n, 3000 + L, case)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=31)
+    o = O.invert(s, cube, guess)
+    g = ctx.invert(cube)
+    _compare(g, o, "%s L%d" % (case, L), TOL_GEN, 0.98)
+    # parameters that are not free keep the guess
+    for i in range(9, 11):
+        if not CASES[case][0][i]:
+            assert np.all(g["model"][:, i] == guess[i])
+
+
+def test_invert_general_mixed():
+    """The mixed form of the 112-sample v_mac inversion against the FP64
+    oracle, with the mixed form's own tolerances."""
+    n = 256
+    say("case general invert mixed mac_nt10 L=112 npix=%d" % n)
+    s, ctx, guess, sl = _pair("mac_nt10", 112, True, precision=pmilos.PM_MIXED)
+    truths = _models(n, 3112, "mac_nt10")
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=31)
+    o = O.invert(s, cube, guess)
+    g = ctx.invert(cube)
+    tol = np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 1e-2, 1e-2, 1e-2, 1e-2])
+    _compare(g, o, "mixed mac_nt10 L112", tol, 0.97)
+
+
+def test_invert_per_pixel_stray_light():
+    """A stray-light profile per pixel (the 4-D stray-light cube of
+    milos.c:986-996): each pixel against the oracle set up with its own
+    profile, and a cube repeating the context's profile gives exactly the
+    context-profile inversion."""
+    n, L = 48, 30
+    say("case per-pixel stray light L=%d npix=%d" % (L, n))
+    s0, ctx, guess, sl = _pair("ff_nt10_sl", L, True)
+    truths = _models(n, 4000, "ff_nt10_sl")
+    lam = O.grid(*GRIDS[L])
+    G = O.gauss_psf(60.0, lam)
+    rng = np.random.default_rng(41)
+    scl = (sl[None] * rng.uniform(0.8, 1.2, (n, 1, 1))).astype(np.float32)
+    fix = CASES["ff_nt10_sl"][0]
+    cube = np.zeros((n, 4, L), dtype=np.float32)
+    outs = []
+    for i in range(n):
+        si = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=fix, slight=scl[i].astype(np.float64))
+        cube[i] = add_noise(O.synth(si, truths[i])[None], truths[i:i + 1], 1e-3, seed=50 + i)[0]
+        outs.append((si, cube[i:i + 1]))
+    g = ctx.invert(cube, slight=scl)
+    o = {k: [] for k in ("model", "chi2", "iters")}
+    for si, c in outs:
+        r = O.invert(si, c, guess)
+        for k in o:
+            o[k].append(r[k][0])
+    o = {k: np.array(v) for k, v in o.items()}
+    _compare(g, o, "per-pixel slight", TOL_GEN, 0.95)
+    rep = np.repeat(sl[None], n, axis=0)
+    a = ctx.invert(cube, slight=rep)
+    b = ctx.invert(cube)
+    assert np.array_equal(a["model"], b["model"]) and np.array_equal(a["iters"], b["iters"])

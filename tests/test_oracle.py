@@ -204,6 +204,54 @@ def test_response_functions_finite_differences():
     np.testing.assert_allclose(rf_ref[1:, 8], sp[1:] / m[8], rtol=1e-12, atol=1e-18)
 
 
+@pytest.mark.parametrize("psf", [False, True])
+def test_response_functions_mac_and_filling_factor(psf):
+    """NTERMS 11 with macroturbulence and stray light, with and without a
+    PSF (which folds into the kernel and its derivative alike, so the same
+    identities hold).  The filling
+    factor row is the exact derivative Ssl - slight (me_der.c:499-525) and
+    every other row carries the factor alpha.  The v_mac row is the
+    circular convolution with the kernel derivative gmac/v_mac (a^2 - 1)
+    (me_der.c:328-329), which uses the continuous normalisation <a^2> = 1;
+    the derivative of the discretely normalised kernel is
+    gmac/v_mac (a^2 - <a^2>), so the finite difference equals the row plus
+    alpha Ssl (1 - <a^2>) / v_mac, with <a^2> the kernel's own second
+    moment and Ssl the spectrum before the stray-light mix."""
+    lam = O.grid(FE6173, -600, 25, 600)
+    L = len(lam)
+    rng = np.random.default_rng(5)
+    slight = rng.uniform(0.0, 0.02, (4, L)).astype(np.float32).astype(np.float64)
+    slight[0] += 0.9
+    G = O.gauss_psf(60.0, lam) if psf else None
+    s = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=[1] * 11, guess_mac=0.5, slight=slight)
+    s.exact_rho_deriv = 1
+    assert s.nterms == 11
+    steps = [1e-4, 1e-2, 1e-4, 1e-7, 1e-5, 1e-4, 1e-4, 1e-6, 1e-6, 1e-5, 1e-6]
+    for m in (np.array([8., 1200., 0.7, 0.03, 0.2, 60., 30., 0.25, 0.72, 1.2, 0.8]),
+              np.array([25., 300., -1.1, 0.045, 0.08, 150., 160., 0.1, 0.9, 0.4, 0.65])):
+        sp, rf = O.synth(s, m, rf=True)
+        g = np.zeros(L)
+        O.lib().orc_gauss_mac(m[9], lam, L, FE6173, 0, g)
+        a = (lam - lam[L // 2]) / (FE6173 * m[9] / 2.99792458e5)
+        a2 = float(np.sum(g * a * a))
+        for p in range(11):
+            fd = _fd_rf(s, m, p, steps[p] * max(1.0, abs(m[p])))
+            want = rf[:, p]
+            if p == 9:
+                want = rf[:, 9] + (sp - slight * (1 - m[10])) * (1.0 - a2) / m[9]
+            scale = np.abs(fd).max()
+            tol = 2e-3 if p in (1, 2, 3, 4) else 1e-6
+            assert np.abs(fd - want).max() <= tol * scale, (p, np.abs(fd - want).max() / scale)
+        # the correction is far above the tolerance for the narrow kernel
+        # (0.4 km/s: sigma 8 mA on a 25 mA grid): dropping it fails there
+        if m[9] < 1:
+            fd9 = _fd_rf(s, m, 9, 1e-5 * m[9])
+            assert np.abs(fd9 - rf[:, 9]).max() > 1e-3 * np.abs(fd9).max()
+        # the filling-factor row exactly
+        sp_pure = (sp - slight * (1 - m[10])) / m[10]
+        np.testing.assert_allclose(rf[:, 10], sp_pure - slight, rtol=1e-12, atol=1e-14)
+
+
 def test_response_functions_psf_is_row_convolution():
     """With a PSF, QUV rows are the zero-padded correlation of the raw rows and
     I rows use the Ic trick with each row's own max(first, last); the S0 row is
