@@ -133,9 +133,12 @@ def main():
     out_i = torch.empty(npix, dtype=torch.int32, device=dev)
     out_a = torch.empty(npix, dtype=torch.int32, device=dev)
     if args.copy_ref_mb:
-        src = torch.ones(args.copy_ref_mb << 20, dtype=torch.uint8, device=dev)
+        # a plain streaming kernel of known traffic (reads and writes
+        # copy_ref_mb MiB once each) for scaling the HBM byte counters
+        src = torch.ones((args.copy_ref_mb << 20) // 4, dtype=torch.float32, device=dev)
         dst = torch.empty_like(src)
-        dst.copy_(src)
+        torch.neg(src, out=dst)
+        torch.cuda.synchronize()
         del src, dst
     torch.cuda.synchronize()
 

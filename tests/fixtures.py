@@ -65,3 +65,19 @@ def setup_l30(psf=True, **kw):
 
 def oracle_cube(s, models):
     return np.stack([O.synth(s, m) for m in models])
+
+
+def read_mod(path):
+    """Init_Model from a .mod file (the reference's model text format)."""
+    keys = {"eta_0": 0, "magnetic field": 1, "LOS velocity": 2, "Doppler width": 3,
+            "damping": 4, "gamma": 5, "phi": 6, "S_0": 7, "S_1": 8, "v_mac": 9,
+            "filling factor": 10}
+    m = np.zeros(11)
+    for line in open(path):
+        if ":" not in line:
+            continue
+        name, val = line.split(":", 1)
+        for k, i in keys.items():
+            if k in name:
+                m[i] = float(val.split()[0])
+    return m

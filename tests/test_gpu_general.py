@@ -11,7 +11,7 @@ import pytest
 import oracle as O
 import pmilos
 from fixtures import (FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models,
-                      oracle_cube, read_fits_2d)
+                      oracle_cube, read_fits_2d, read_mod)
 
 pytestmark = pytest.mark.gpu
 
@@ -89,17 +89,7 @@ def _err_rf(a, b):
 def _golden():
     slight = read_fits_2d(os.path.join(GOLDEN, "straylight.fits"))
     lam = O.grid(FE6173, -600, 25, 600)
-    keys = {"eta0": 0, "magnetic field": 1, "LOS velocity": 2, "Doppler width": 3,
-            "Damping": 4, "gamma": 5, "phi": 6, "S_0": 7, "S_1": 8, "v_mac": 9,
-            "filling factor": 10}
-    model = np.zeros(11)
-    for line in open(os.path.join(GOLDEN, "obs.mod")):
-        if ":" not in line:
-            continue
-        name, val = line.split(":", 1)
-        for k, i in keys.items():
-            if k in name:
-                model[i] = float(val.split()[0])
+    model = read_mod(os.path.join(GOLDEN, "obs.mod"))
     gold = np.loadtxt(os.path.join(GOLDEN, "synthesis_results.per"))[:, 2:].T
     return slight, lam, model, gold
 

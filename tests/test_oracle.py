@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 import oracle as O
-from fixtures import (FE6173, GUESS, TRIPLET, GOLDEN, add_noise, draw_models,
+from fixtures import (FE6173, GUESS, TRIPLET, GOLDEN, add_noise, draw_models, read_mod,
                       oracle_cube, read_fits_2d, setup_l30)
 
 VEC = np.load(os.path.join(GOLDEN, "ref_core_vectors.npz"))
@@ -15,28 +15,13 @@ VEC = np.load(os.path.join(GOLDEN, "ref_core_vectors.npz"))
 
 # ------------------------------------------------ reference fixtures / code
 
-def _read_mod(path):
-    keys = {"eta_0": 0, "magnetic field": 1, "LOS velocity": 2, "Doppler width": 3,
-            "damping": 4, "gamma": 5, "phi": 6, "S_0": 7, "S_1": 8, "v_mac": 9,
-            "filling factor": 10}
-    m = np.zeros(11)
-    for line in open(path):
-        if ":" not in line:
-            continue
-        name, val = line.split(":", 1)
-        for k, i in keys.items():
-            if k in name:
-                m[i] = float(val.split()[0])
-    return m
-
-
 def test_golden_synthesis(golden_dir):
     """test/synthesis.mtrol -> test/synthesis_results.per (manual p.18): 49
     wavelengths, stray light with filling factor 0.85, no PSF."""
     slight = read_fits_2d(os.path.join(golden_dir, "straylight.fits"))
     assert slight.shape == (4, 49)
     lam = O.grid(FE6173, -600, 25, 600)
-    model = _read_mod(os.path.join(golden_dir, "obs.mod"))
+    model = read_mod(os.path.join(golden_dir, "obs.mod"))
     s = O.make_setup(FE6173, TRIPLET, lam, psf=None, fix=[1] * 9 + [0, 1],
                      guess_mac=model[9], slight=slight)
     spec = O.synth(s, model)
