@@ -11,8 +11,11 @@ as float32.  Inversion: 9 free parameters, <= 50 LM iterations, the run/
 guess.mod initial model plus classical estimates per pixel.
 
 One step = one inversion pass over the whole cube, inputs resident in HBM.
+After the FP64 and mixed passes, a bounded streaming run (pmilos.serving:
+Poisson arrivals of scans of 32^2..2048^2 pixels at 6..112 wavelengths
+through the serving loop, mixed form) gives p50/p99 scan latency.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--npix N]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--npix N] [--serve-scans S]
 """
 import argparse
 import json
@@ -96,6 +99,9 @@ def main():
                     help="first run a plain device copy of this many MiB (the reference "
                          "kernel for scaling HBM traffic counters)")
     ap.add_argument("--cpu-per-core", type=int, default=4000)
+    ap.add_argument("--serve-scans", type=int, default=32,
+                    help="scans of the streaming request loop (0 = skip it)")
+    ap.add_argument("--serve-util", type=float, default=0.5)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding "
@@ -209,6 +215,12 @@ def main():
             "p99_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
                                              [float("%.3g" % v) for v in np.percentile(d, 99, 0)])),
         }
+    if args.serve_scans > 0:
+        from pmilos import serving
+        del stokes, out_m, out_c, out_i, out_a
+        torch.cuda.empty_cache()
+        line["serving"] = serving.serve_bench(pmilos.PM_MIXED, args.serve_scans,
+                                              args.serve_util)
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

@@ -22,9 +22,16 @@
  *   pm_grid          wavelength axis milos.c:173-197 (readMallaGrid output).
  *   pm_psf_gaussian  fgauss_WL, fgauss.c:79-154, as called at milos.c:269.
  *   pm_nterms        NTERMS, milos.c:154-157.
+ *   pm_server_*, pm_submit, pm_poll, pm_wait
+ *                    the MPI master/worker farm that hands cubes to ranks
+ *                    (pmilos.c:1181-1336) and the time-series loop that
+ *                    takes cubes as they appear (pmilos.c:2047-2103): scans
+ *                    are submitted as they arrive and share one GPU.
  */
 #ifndef PMILOS_H
 #define PMILOS_H
+
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -128,6 +135,44 @@ int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_sligh
  * pseudo-inverse fallback in the last inversion on this context (diagnostic;
  * synchronises the context's last stream). */
 int pm_last_fallback_count(pm_ctx *ctx, long long *count);
+
+/* ------------------------------------------------------------ serving
+ *
+ * A server takes scans (Stokes cubes of any size, on any context of its
+ * device) as they arrive and inverts them concurrently.  Scans are admitted
+ * while the device bytes of the admitted ones stay within hbm_budget (the
+ * first queued scan that fits; any scan when none is admitted), uploaded,
+ * split into pixel chunks that are run least-remaining-work first on
+ * several streams, and their results are copied back into the caller's
+ * buffers.  A scan's results are the same bits as pm_invert / pm_invert_sl
+ * give for it alone.  The caller keeps its buffers alive and unchanged
+ * until pm_wait returns for the ticket.  One host thread may submit while
+ * others wait. */
+typedef struct pm_server pm_server;
+
+typedef struct {
+    size_t hbm_budget;           /* device bytes of admitted scans; 0 = half of the free memory */
+    int streams;                 /* compute streams, 1..8 (0 = 3) */
+    int chunk_px;                /* pixels per chunk at 30 wavelengths, scaled by 30/L (0 = 65536) */
+} pm_server_opts;
+
+int  pm_server_create(const pm_server_opts *opts, pm_server **out);
+/* Finishes every submitted scan, then frees the server. */
+void pm_server_destroy(pm_server *srv);
+
+/* Queues a scan; arguments as pm_invert_sl.  PM_ENOMEM when the scan alone
+ * needs more than the budget. */
+int pm_submit(pm_server *srv, pm_ctx *ctx, const float *stokes, const float *slight, int npix,
+              double *models, float *chi2, int *iters, int *accepted, float *fit,
+              long long *ticket);
+
+/* 1 when the scan's results are in the caller's buffers, 0 while not. */
+int pm_poll(pm_server *srv, long long ticket);
+
+/* Blocks until the scan is done and retires the ticket; latency_s
+ * (nullable) = submission to results in host memory, in seconds.  Returns
+ * the scan's status. */
+int pm_wait(pm_server *srv, long long ticket, double *latency_s);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,22 @@
+// pm_internal.h -- declarations shared by the library's own sources (not
+// part of the C ABI).
+#ifndef PM_INTERNAL_H
+#define PM_INTERNAL_H
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+struct pm_ctx;
+
+// Queues one inversion pass of npix pixels on stream st with the given
+// work-queue counters (pm_counters_bytes() of device memory) and fallback
+// counter: launches on different streams need their own counters.
+int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
+                     double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
+                     float *d_fit, int *d_counters, unsigned long long *d_fallbacks,
+                     hipStream_t st);
+int pm_ctx_nlambda(const pm_ctx *ctx);
+int pm_ctx_device(const pm_ctx *ctx);
+size_t pm_counters_bytes();
+
+#endif

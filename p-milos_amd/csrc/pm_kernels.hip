@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include "pm_device.h"
+#include "pm_internal.h"
 #include "../../include/pmilos.h"
 
 #define PM_BLOCK 256
@@ -1047,15 +1048,14 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     return PM_DISPATCH_L(ctx, gen, L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
 }
 
-extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_slight,
-                                   int npix, double *d_models, float *d_chi2, int *d_iters,
-                                   int *d_accepted, float *d_fit, void *stream)
+int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
+                     double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
+                     float *d_fit, int *d_counters, unsigned long long *d_fallbacks,
+                     hipStream_t st)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
     if (!d_stokes || !d_models || !d_chi2 || !d_iters) return PM_EINVAL;
-    hipStream_t st = (hipStream_t)stream;
-    ctx->last_stream = st;
     InvArgs a;
     a.c = ctx->d;
     a.stokes = d_stokes;
@@ -1066,12 +1066,27 @@ extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const flo
     a.iters = d_iters;
     a.accepted = d_accepted;
     a.fit = d_fit;
-    a.counters = ctx->counters;
-    a.fallbacks = ctx->fallbacks;
+    a.counters = d_counters;
+    a.fallbacks = d_fallbacks;
     a.slight = d_slight;
     const int L = ctx->h.L;
     const bool gen = needs_general(ctx, d_slight != nullptr);
     return PM_DISPATCH_L(ctx, gen, L, launch_invert, ctx, a, st);
+}
+
+int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
+int pm_ctx_device(const pm_ctx *ctx) { return ctx->device; }
+size_t pm_counters_bytes() { return sizeof(int) * PM_NSLICE * PM_CSTRIDE; }
+
+extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_slight,
+                                   int npix, double *d_models, float *d_chi2, int *d_iters,
+                                   int *d_accepted, float *d_fit, void *stream)
+{
+    if (!ctx) return PM_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    ctx->last_stream = st;
+    return pm_invert_launch(ctx, d_stokes, d_slight, npix, d_models, d_chi2, d_iters, d_accepted,
+                            d_fit, ctx->counters, ctx->fallbacks, st);
 }
 
 extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,

@@ -39,6 +39,10 @@ class Config(C.Structure):
                 ("slight", C.POINTER(C.c_float))]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", C.c_size_t), ("streams", C.c_int), ("chunk_px", C.c_int)]
+
+
 def lib():
     """The loaded HIP library; raises if it was not built."""
     global _lib
@@ -63,6 +67,13 @@ def lib():
         L.pm_invert_sl.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp]
         L.pm_invert_sl_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.pm_last_fallback_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.pm_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
+        L.pm_server_destroy.argtypes = [vp]
+        L.pm_server_destroy.restype = None
+        L.pm_submit.argtypes = [vp, vp, vp, vp, C.c_int, vp, vp, vp, vp, vp,
+                                C.POINTER(C.c_longlong)]
+        L.pm_poll.argtypes = [vp, C.c_longlong]
+        L.pm_wait.argtypes = [vp, C.c_longlong, C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -182,3 +193,55 @@ class Context:
         v = C.c_longlong(0)
         _check(lib().pm_last_fallback_count(self.h, C.byref(v)))
         return v.value
+
+
+class Server:
+    """Serving loop on the device (pm_server): scans are submitted as they
+    arrive, share the GPU, and are collected with wait()."""
+
+    def __init__(self, hbm_budget=0, streams=0, chunk_px=0):
+        o = ServerOpts(hbm_budget, streams, chunk_px)
+        h = C.c_void_p()
+        _check(lib().pm_server_create(C.byref(o), C.byref(h)))
+        self.h = h
+        self._pending = {}
+
+    def submit(self, ctx, stokes, slight=None, want_fit=False):
+        """Queues a scan; returns a ticket.  The result buffers are allocated
+        here and kept with the ticket."""
+        st = np.ascontiguousarray(stokes, dtype=np.float32)
+        n = st.shape[0]
+        sl = None if slight is None else np.ascontiguousarray(slight, dtype=np.float32)
+        out = dict(model=np.zeros((n, NPAR)), chi2=np.zeros(n, dtype=np.float32),
+                   iters=np.zeros(n, dtype=np.int32), accepted=np.zeros(n, dtype=np.int32),
+                   fit=np.zeros((n, 4, ctx.L), dtype=np.float32) if want_fit else None)
+        t = C.c_longlong(0)
+        _check(lib().pm_submit(self.h, ctx.h, st.ctypes.data, _ptr(sl), n,
+                               out["model"].ctypes.data, out["chi2"].ctypes.data,
+                               out["iters"].ctypes.data, out["accepted"].ctypes.data,
+                               _ptr(out["fit"]), C.byref(t)))
+        self._pending[t.value] = (st, sl, out)
+        return t.value
+
+    def poll(self, ticket):
+        return _check(lib().pm_poll(self.h, ticket)) == 1
+
+    def wait(self, ticket):
+        """Blocks until the scan is done: (results dict, latency in s)."""
+        lat = C.c_double(0)
+        rc = lib().pm_wait(self.h, ticket, C.byref(lat))
+        _, _, out = self._pending.pop(ticket)
+        _check(rc)
+        return out, lat.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pm_server_destroy(self.h)
+            self.h = None
+            self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

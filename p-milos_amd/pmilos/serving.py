@@ -1,0 +1,189 @@
+"""Streaming request loop on one GPU: scans arrive as a Poisson process and
+go through pmilos.Server, which admits them by HBM budget and runs them
+concurrently.  Reports pixels/s and p50/p99 per-scan latency (arrival to
+results in host memory).
+
+Scan mix: sizes {32x32, 256x256, 1024x1024, 2048x2048} pixels and samplings
+{6, 14, 30, 112} wavelengths (Fe I 6173.3356 A at -172.5..172.5 @ 69 mA,
+-260..260 @ 40 mA, -290..290 @ 20 mA; Fe I 6302.4936 A at -1193.25..1193.25
+@ 21.5 mA with v_mac free, 10 parameters), Gaussian PSF of 60 mA, models
+drawn as the benchmark field's (seed 500 + scan kind), noise 1e-3 Ic.  One
+synthetic cube per (size, sampling) is made on the GPU and reused by the
+scans of that kind.
+
+    python -m pmilos.serving [--scans N] [--util U] [--precision mixed|fp64]
+"""
+import argparse
+import json
+import threading
+import time
+
+import numpy as np
+
+import pmilos
+
+FE6173, FE6302 = 6173.3356, 6302.4936
+TRIPLET = (2, 1, 1.0, 2, 2, 0.0)
+GUESS = np.array([20, 1100, 0.2, 0.03, 0.05, 120, 150, 0.35, 0.5, 0, 1.0])
+SAMPLINGS = {6: (FE6173, -172.5, 69, 172.5), 14: (FE6173, -260, 40, 260),
+             30: (FE6173, -290, 20, 290), 112: (FE6302, -1193.25, 21.5, 1193.25)}
+SIZES = (32, 256, 1024, 2048)
+
+
+def draw_models(n, seed, mac):
+    rng = np.random.default_rng(seed)
+    B = rng.uniform(0, 2500, n)
+    gm = rng.uniform(0, 180, n)
+    az = rng.uniform(0, 180, n)
+    vl = rng.normal(0, 1.5, n)
+    eta = np.exp(rng.uniform(0, np.log(30), n))
+    dop = rng.uniform(0.020, 0.050, n)
+    aa = rng.uniform(0, 0.5, n)
+    S0 = rng.uniform(0.1, 0.4, n)
+    S1 = 1 - S0 + rng.uniform(-0.05, 0.05, n)
+    vmac = rng.uniform(0, 2, n) if mac else np.zeros(n)
+    return np.stack([eta, B, vl, dop, aa, gm, az, S0, S1, vmac, np.ones(n)], 1)
+
+
+def contexts(precision):
+    """One resident context per sampling (line, grid, PSF, Zeeman pattern)."""
+    out = {}
+    for L, g in SAMPLINGS.items():
+        lam = pmilos.grid(*g)
+        G = pmilos.psf_gaussian(60.0, lam)
+        guess, fix = GUESS.copy(), [1] * 9 + [0, 0]
+        if L == 112:
+            guess[9], fix = 0.5, [1] * 10 + [0]
+        out[L] = pmilos.Context(g[0], TRIPLET, lam, psf=G, guess=guess, fix=fix, miter=50,
+                                precision=precision)
+    return out
+
+
+def make_cube(ctx, npix, seed, mac):
+    """Synthetic float32 cube [npix][4][L] in host memory: the library's FP64
+    synthesis of seeded models plus seeded noise, made on the GPU."""
+    import torch
+    dev = torch.device("cuda", torch.cuda.current_device())
+    L = ctx.L
+    m = torch.from_numpy(draw_models(npix, seed, mac)).to(dev)
+    spec = torch.empty((npix, 4, L), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream()
+    ctx.synth_ptrs(m.data_ptr(), npix, spec.data_ptr(), 0, st.cuda_stream)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(seed + 1)
+    ic = (m[:, 7] + m[:, 8]).view(-1, 1, 1)
+    spec += torch.randn(spec.shape, dtype=torch.float64, device=dev, generator=gen) * 1e-3 * ic
+    cube = spec.to(torch.float32).cpu().numpy()
+    del spec, m
+    return cube
+
+
+def calibrate(ctxs, pool):
+    """Pixels/s of each context alone on its largest pooled cube (a bounded
+    slice), used to size the arrival rate for a target utilisation."""
+    rate = {}
+    for L, ctx in ctxs.items():
+        cube = pool[(max(s for s, l in pool if l == L), L)]
+        n = min(len(cube), 262144 if L <= 30 else 65536)
+        ctx.invert(cube[:4096])
+        t = time.perf_counter()
+        ctx.invert(cube[:n])
+        rate[L] = n / (time.perf_counter() - t)
+    return rate
+
+
+def run(server, ctxs, pool, kinds, rate_hz, seed=50):
+    """Submits len(kinds) scans at Poisson arrival times; returns per-scan
+    (kind, npix, latency) and the wall time from first arrival to last
+    completion."""
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / rate_hz, len(kinds))
+    tickets = []
+    done = threading.Event()
+
+    def submitter():
+        t_next = time.perf_counter()
+        for k, gap in zip(kinds, gaps):
+            t_next += gap
+            d = t_next - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            size, L = k
+            tickets.append((k, server.submit(ctxs[L], pool[k])))
+        done.set()
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=submitter)
+    th.start()
+    res = []
+    i = 0
+    while True:
+        if i < len(tickets):
+            k, t = tickets[i]
+            _, lat = server.wait(t)
+            res.append((k, k[0] * k[0], lat))
+            i += 1
+        elif done.is_set() and i >= len(tickets):
+            break
+        else:
+            time.sleep(0.001)
+    th.join()
+    return res, time.perf_counter() - t0
+
+
+def summarize(res, wall):
+    lat = np.array([r[2] for r in res])
+    px = sum(r[1] for r in res)
+    out = {"scans": len(res), "pixels_per_s": round(px / wall, 1),
+           "p50_s": round(float(np.percentile(lat, 50)), 4),
+           "p99_s": round(float(np.percentile(lat, 99)), 4), "per_kind": {}}
+    for k in sorted(set(r[0] for r in res)):
+        l = np.array([r[2] for r in res if r[0] == k])
+        out["per_kind"]["%dx%d_L%d" % (k[0], k[0], k[1])] = {
+            "n": len(l), "p50_s": round(float(np.percentile(l, 50)), 4),
+            "p99_s": round(float(np.percentile(l, 99)), 4)}
+    return out
+
+
+def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SIZES,
+                samplings=tuple(SAMPLINGS)):
+    ctxs = contexts(precision)
+    kinds_all = [(s, L) for s in sizes for L in samplings]
+    pool = {}
+    for i, (s, L) in enumerate(kinds_all):
+        pool[(s, L)] = make_cube(ctxs[L], s * s, 500 + i, L == 112)
+    rate = calibrate(ctxs, pool)
+    rng = np.random.default_rng(seed)
+    kinds = [kinds_all[j] for j in rng.integers(0, len(kinds_all), scans)]
+    mean_service = np.mean([s * s / rate[L] for s, L in kinds_all])
+    arrival = util / mean_service
+    srv = pmilos.Server()
+    try:
+        srv.wait(srv.submit(ctxs[30], pool[(32, 30)]))   # first use of its streams
+        res, wall = run(srv, ctxs, pool, kinds, arrival, seed)
+    finally:
+        srv.close()
+    out = summarize(res, wall)
+    out.update({"arrival_rate_hz": round(arrival, 4), "target_utilisation": util,
+                "mean_service_s": round(float(mean_service), 4),
+                "alone_pixels_per_s": {str(L): round(r, 1) for L, r in rate.items()},
+                "precision": "mixed" if precision == pmilos.PM_MIXED else "fp64"})
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--scans", type=int, default=48)
+    ap.add_argument("--util", type=float, nargs="+", default=[0.5])
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--seed", type=int, default=50)
+    args = ap.parse_args()
+    import torch
+    torch.cuda.set_device(0)
+    prec = pmilos.PM_MIXED if args.precision == "mixed" else pmilos.PM_FP64
+    for u in args.util:
+        print(json.dumps(serve_bench(prec, args.scans, u, args.seed)), flush=True)
+
+
+if __name__ == "__main__":
+    main()
