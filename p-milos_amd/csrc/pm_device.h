@@ -146,6 +146,30 @@ __device__ __forceinline__ int group_sum_int(int v)
     return v;
 }
 
+template <int W>
+__device__ __forceinline__ int group_min_int(int v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, W));
+    return v;
+}
+
+template <int W>
+__device__ __forceinline__ int group_max_int(int v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, W));
+    return v;
+}
+
+template <int W>
+__device__ __forceinline__ double group_max(double v)
+{
+#pragma unroll
+    for (int o = W / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, W));
+    return v;
+}
+
 // Orders LDS traffic between lanes of one wavefront (groups never span
 // waves, so no workgroup barrier is needed).
 __device__ __forceinline__ void wave_sync()

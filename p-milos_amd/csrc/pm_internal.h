@@ -16,6 +16,9 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
                      float *d_fit, int *d_counters, unsigned long long *d_fallbacks,
                      hipStream_t st);
 int pm_ctx_nlambda(const pm_ctx *ctx);
+// Estimated work per pixel and iteration, relative (for sizing and ordering
+// chunks of work).
+double pm_ctx_cost(const pm_ctx *ctx);
 int pm_ctx_device(const pm_ctx *ctx);
 size_t pm_counters_bytes();
 

@@ -125,11 +125,13 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
 }
 
 // Circular convolution with the kernel h (LDS, length L), rotated so that
-// out[m] = conv[(m + L/2) mod L] (convCircular, convolution.c:126-153); the
-// rows are staged in the middle of the PSF row buffer, leaving its zero
-// margins untouched.
+// out[m] = conv[(m + L/2) mod L] (convCircular, convolution.c:126-153), over
+// the kernel's taps [jlo, jhi): the others are below the rounding of the
+// sum (see Taps).  The rows are staged in the middle of the PSF row buffer,
+// leaving its zero margins untouched.
 template <typename T, int W, int NLPL, int R>
-__device__ __forceinline__ void convC(T *pad, const T *h, int t, int L, T x[R][NLPL])
+__device__ __forceinline__ void convC(T *pad, const T *h, int jlo, int jhi, int t, int L,
+                                      T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
     constexpr int PADW = 2 * W * NLPL;
@@ -151,12 +153,13 @@ __device__ __forceinline__ void convC(T *pad, const T *h, int t, int L, T x[R][N
             A acc[R];
 #pragma unroll
             for (int r = 0; r < R; r++) acc[r] = 0;
-            for (int j = 0; j < L; j++) {
-                int idx = i - j;
-                if (idx < 0) idx += L;
+            int idx = i - jlo;
+            if (idx < 0) idx += L;
+            for (int j = jlo; j < jhi; j++) {
                 A hj = (A)h[j];
 #pragma unroll
                 for (int r = 0; r < R; r++) acc[r] += hj * (A)pad[r * PADW + half + idx];
+                idx = idx == 0 ? L - 1 : idx - 1;
             }
 #pragma unroll
             for (int r = 0; r < R; r++) x[r][k] = T(acc[r]);
@@ -165,13 +168,40 @@ __device__ __forceinline__ void convC(T *pad, const T *h, int t, int L, T x[R][N
     wave_sync();
 }
 
+// Tap windows of the two macroturbulence kernels: [lo, hi) holds every tap
+// whose magnitude reaches 1e-17 (FP64 form) / 1e-9 (FP32 form) of the
+// kernel's largest; the rest are below the rounding of the sums they enter.
+struct Taps {
+    int mlo, mhi, dlo, dhi;
+};
+
+template <typename T, int W, int NLPL>
+__device__ __forceinline__ void tap_window(const T v[NLPL], int t, int L, int &lo, int &hi)
+{
+    double mx = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++)
+        if (t + k * W < L) mx = fmax(mx, fabs((double)v[k]));
+    mx = group_max<W>(mx);
+    const double thr = mx * (sizeof(T) == 8 ? 1e-17 : 1e-9);
+    int a = L, b = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L && fabs((double)v[k]) >= thr) { a = min(a, lam); b = max(b, lam + 1); }
+    }
+    lo = group_min_int<W>(a);
+    hi = group_max_int<W>(b);
+    if (lo >= hi) { lo = 0; hi = 0; }
+}
+
 // Macroturbulence kernel of model m and its v_mac derivative (fgauss,
 // fgauss.c:16-66; the derivative as me_der.c:328-329,410), both correlated
 // with the PSF when there is one (mil_sinrf.c:348-350, me_der.c:412-416);
-// written to gm / gd in LDS.
+// written to gm / gd in LDS, with their tap windows.
 template <typename T, int W, int NLPL>
 __device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const double *m,
-                                            T *pad, int t, int L, T *gm, T *gd)
+                                            T *pad, int t, int L, T *gm, T *gd, Taps &tp)
 {
     const double MC = m[9];
     const double ild = (c->lambda0 * MC) / 2.99792458e5;
@@ -204,6 +234,8 @@ __device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const
         int lam = t + k * W;
         if (lam < L) { gm[lam] = x[0][k]; gd[lam] = x[1][k]; }
     }
+    tap_window<T, W, NLPL>(x[0], t, L, tp.mlo, tp.mhi);
+    tap_window<T, W, NLPL>(x[1], t, L, tp.dlo, tp.dhi);
     wave_sync();
 }
 
@@ -213,6 +245,7 @@ struct Extra {
     T Sraw[4][NLPL];     // before any convolution (spectra_mac)
     T Ssl[4][NLPL];      // after convolution, before the stray-light mix
     bool mac;            // macroturbulence applied (v_mac > 1e-4)
+    Taps tp;             // tap windows of the macroturbulence kernels
 };
 
 // Zero the response-function row of a fixed parameter,
@@ -256,8 +289,8 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
         ex.mac = m[9] > 0.0001;                                    // mil_sinrf.c:288
     }
     if (ex.mac) {
-        mac_kernels<T, W, NLPL>(c, m, pad, t, L, gm, gd);
-        convC<T, W, NLPL, 4>(pad, gm, t, L, S);                    // :353-354
+        mac_kernels<T, W, NLPL>(c, m, pad, t, L, gm, gd, ex.tp);
+        convC<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, S);   // :353-354
     } else if (c->use_psf) {
         convR<T, W, NLPL, 4>(c, pad, t, L, S);                     // :387-402
     }
@@ -307,7 +340,7 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
         for (int k = 0; k < NLPL; k++)
 #pragma unroll
             for (int s = 0; s < 4; s++) x[s][k] = ex.Sraw[s][k];
-        convC<T, W, NLPL, 4>(pad, gd, t, L, x);
+        convC<T, W, NLPL, 4>(pad, gd, ex.tp.dlo, ex.tp.dhi, t, L, x);
 #pragma unroll
         for (int k = 0; k < NLPL; k++)
 #pragma unroll
@@ -326,7 +359,7 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
 #pragma unroll
                     for (int s = 0; s < 4; s++) x[4 * u + s][k] = D[k][s][p];
             }
-            if (mac) convC<T, W, NLPL, R>(pad, gm, t, L, x);
+            if (mac) convC<T, W, NLPL, R>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
             else convR<T, W, NLPL, R>(c, pad, t, L, x);
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
@@ -1075,6 +1108,17 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
 }
 
 int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
+
+double pm_ctx_cost(const pm_ctx *ctx)
+{
+    // per-iteration work of one pixel, relative: the sample-wise terms, the
+    // convolutions over the PSF taps (or about half of the samples when
+    // macroturbulence is on) for every response row, the NT^3 solve
+    const PmConst &c = ctx->h;
+    const bool mac = c.guess[9] > 0.0001 || c.fix[9];
+    const double taps = mac ? 0.4 * c.L : (c.use_psf ? c.ghi64 - c.glo64 : 0);
+    return c.L * (400.0 + 10.0 * c.NT * taps) + 10.0 * c.NT * c.NT * c.NT;
+}
 int pm_ctx_device(const pm_ctx *ctx) { return ctx->device; }
 size_t pm_counters_bytes() { return sizeof(int) * PM_NSLICE * PM_CSTRIDE; }
 

@@ -3,19 +3,22 @@
 // any time and share the GPU.
 //
 // Host side only.  Two threads per server:
+//   dispatcher  splits every admitted scan into pixel chunks of about the
+//               same estimated work and keeps each compute stream busy with
+//               one chunk, always taking the next chunk from the admitted
+//               scan with the least work left (shortest remaining work
+//               first): a small scan that arrives behind a large one waits
+//               for one chunk per stream, not for the large scan.  A chunk's
+//               Stokes samples are uploaded on its stream just before its
+//               kernel.  Chunks of different scans run at the same time: a
+//               persistent kernel's workgroups retire as its work queue
+//               drains and the next chunk's take their place.
 //   stager      admits queued scans while the device bytes of the admitted
-//               ones stay within the HBM budget (first queued scan that
-//               fits), uploads their Stokes cube (and stray light), and
-//               downloads the results of finished scans, freeing their
-//               budget.  Downloads go first: they release memory.
-//   dispatcher  splits every admitted scan into pixel chunks and keeps each
-//               compute stream fed with at most DEPTH chunks, always taking
-//               the next chunk from the admitted scan with the least work
-//               left (shortest remaining work first), so a small scan that
-//               arrives behind a large one is not queued behind all of it.
-//               Chunks of different scans run at the same time on the
-//               device: a persistent kernel's workgroups retire as its work
-//               queue drains and the next chunk's take their place.
+//               ones stay within the HBM budget (the first queued scan that
+//               fits; any scan when none is admitted), and copies the
+//               results of every finished chunk back into the caller's
+//               buffers; a scan is done when its last chunk is back, which
+//               releases its budget.
 // Each compute stream has its own work-queue counters.  A pixel's result
 // does not depend on which chunk, stream or workgroup computes it, so a scan
 // gives the same bits alone and in the mix.
@@ -37,7 +40,6 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
-constexpr int DEPTH = 2;                 // chunks queued per compute stream
 constexpr size_t ALIGN = 256;
 
 size_t up(size_t n, size_t a) { return (n + a - 1) / a * a; }
@@ -80,7 +82,7 @@ public:
     }
 };
 
-enum State { QUEUED, READY, FINISHED, DONE };
+enum State { QUEUED, READY, DONE };
 
 struct Req {
     long long id = 0;
@@ -91,21 +93,22 @@ struct Req {
     int *iters = nullptr, *accepted = nullptr;
     int npix = 0, L = 0, chunk = 0;
     size_t bytes = 0;
-    double cost = 1;                     // relative work per pixel
+    double cost = 1;                     // estimated work per pixel
     // device buffers, carved from one pool block
     char *blk = nullptr;
     float *d_stokes = nullptr, *d_slight = nullptr, *d_chi2 = nullptr, *d_fit = nullptr;
     double *d_models = nullptr;
     int *d_iters = nullptr, *d_acc = nullptr;
-    int next_px = 0, done_px = 0;
+    int next_px = 0;                     // first pixel not yet dispatched
+    int back_px = 0;                     // pixels whose results are in host memory
     int rc = PM_OK;
     State state = QUEUED;
     Clock::time_point t_submit, t_done;
 };
 
-struct Inflight {
+struct Chunk {
     Req *r;
-    int px;
+    int p0, px;
     hipEvent_t ev;
 };
 
@@ -113,7 +116,8 @@ struct Slot {
     hipStream_t s = nullptr;
     int *counters = nullptr;
     unsigned long long *fallbacks = nullptr;
-    std::deque<Inflight> q;
+    bool busy = false;
+    Chunk cur{};
     std::vector<hipEvent_t> spare;
 };
 
@@ -130,13 +134,15 @@ struct pm_server {
     int device = 0;
     size_t budget = 0;
     int chunk_px = 0;
+    double ref_cost = 1;                 // cost of the 30-sample, 9-parameter workload
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Req *> queued;
-    std::vector<Req *> ready;
-    std::deque<Req *> finished;
+    std::vector<Req *> ready;            // admitted, chunks left to dispatch
+    std::deque<Chunk> back;              // finished chunks to copy back
     std::map<long long, Req *> all;
     size_t in_use = 0;
+    int inflight = 0;                    // chunks dispatched, not yet copied back
     bool stop = false;
     long long next_id = 1;
     hipStream_t xfer = nullptr;
@@ -147,12 +153,14 @@ struct pm_server {
     void stager_loop();
     void dispatcher_loop();
     bool admit_one(std::unique_lock<std::mutex> &lk);
-    void finish_one(Req *r);
+    bool copy_back(const Chunk &c);
+    bool launch(Slot &sl, Req *r, std::unique_lock<std::mutex> &lk);
+    void complete(Req *r, int px);
+    bool idle() const { return queued.empty() && ready.empty() && back.empty() && inflight == 0; }
 };
 
-// Admits the first queued scan that fits the budget (any scan when nothing
-// is admitted): allocates and uploads it.  Called with the lock held;
-// returns false when nothing could be admitted.
+// Admits the first queued scan that fits the budget and gives it its
+// device memory.  Called with the lock held; false when none fits.
 bool pm_server::admit_one(std::unique_lock<std::mutex> &lk)
 {
     auto it = std::find_if(queued.begin(), queued.end(), [&](Req *r) {
@@ -163,56 +171,34 @@ bool pm_server::admit_one(std::unique_lock<std::mutex> &lk)
     queued.erase(it);
     in_use += r->bytes;
     lk.unlock();
-    int rc = PM_OK;
     char *p = pool.get(r->bytes);
-    if (!p) rc = PM_ENOMEM;
-    if (rc == PM_OK) {
-        size_t n = r->npix, cube = n * 4 * r->L * sizeof(float);
-        r->blk = p;
-        r->d_stokes = (float *)p; p += up(cube, ALIGN);
-        if (r->slight) { r->d_slight = (float *)p; p += up(cube, ALIGN); }
-        r->d_models = (double *)p; p += up(n * 11 * sizeof(double), ALIGN);
-        r->d_chi2 = (float *)p; p += up(n * sizeof(float), ALIGN);
-        r->d_iters = (int *)p; p += up(n * sizeof(int), ALIGN);
-        r->d_acc = (int *)p; p += up(n * sizeof(int), ALIGN);
-        if (r->fit) r->d_fit = (float *)p;
-        if (hipMemcpyAsync(r->d_stokes, r->stokes, cube, hipMemcpyHostToDevice, xfer) != hipSuccess ||
-            (r->slight && hipMemcpyAsync(r->d_slight, r->slight, cube, hipMemcpyHostToDevice,
-                                         xfer) != hipSuccess) ||
-            hipStreamSynchronize(xfer) != hipSuccess)
-            rc = PM_EDEVICE;
-    }
     lk.lock();
-    if (rc != PM_OK) {
-        r->rc = rc;
-        finished.push_back(r);           // completes with the error
-    } else {
-        r->state = READY;
-        ready.push_back(r);
+    if (!p) {
+        r->rc = PM_ENOMEM;
+        complete(r, r->npix);
+        return true;
     }
+    size_t n = r->npix, cube = n * 4 * r->L * sizeof(float);
+    r->blk = p;
+    r->d_stokes = (float *)p; p += up(cube, ALIGN);
+    if (r->slight) { r->d_slight = (float *)p; p += up(cube, ALIGN); }
+    r->d_models = (double *)p; p += up(n * 11 * sizeof(double), ALIGN);
+    r->d_chi2 = (float *)p; p += up(n * sizeof(float), ALIGN);
+    r->d_iters = (int *)p; p += up(n * sizeof(int), ALIGN);
+    r->d_acc = (int *)p; p += up(n * sizeof(int), ALIGN);
+    if (r->fit) r->d_fit = (float *)p;
+    r->state = READY;
+    ready.push_back(r);
     cv.notify_all();
     return true;
 }
 
-// Downloads a finished scan's results and releases its budget.
-void pm_server::finish_one(Req *r)
+// Counts px pixels of r as back in host memory; the scan is done when all
+// are.  Called with the lock held.
+void pm_server::complete(Req *r, int px)
 {
-    if (r->rc == PM_OK && r->blk) {
-        size_t n = r->npix, cube = n * 4 * r->L * sizeof(float);
-        bool ok = hipMemcpyAsync(r->models, r->d_models, n * 11 * sizeof(double),
-                                 hipMemcpyDeviceToHost, xfer) == hipSuccess &&
-                  hipMemcpyAsync(r->chi2, r->d_chi2, n * sizeof(float), hipMemcpyDeviceToHost,
-                                 xfer) == hipSuccess &&
-                  hipMemcpyAsync(r->iters, r->d_iters, n * sizeof(int), hipMemcpyDeviceToHost,
-                                 xfer) == hipSuccess &&
-                  (!r->accepted || hipMemcpyAsync(r->accepted, r->d_acc, n * sizeof(int),
-                                                  hipMemcpyDeviceToHost, xfer) == hipSuccess) &&
-                  (!r->fit || hipMemcpyAsync(r->fit, r->d_fit, cube, hipMemcpyDeviceToHost,
-                                             xfer) == hipSuccess) &&
-                  hipStreamSynchronize(xfer) == hipSuccess;
-        if (!ok) r->rc = PM_EDEVICE;
-    }
-    std::lock_guard<std::mutex> g(mu);
+    r->back_px += px;
+    if (r->back_px < r->npix) return;
     if (r->blk) pool.put(r->blk);
     r->blk = nullptr;
     in_use -= r->bytes;
@@ -221,23 +207,90 @@ void pm_server::finish_one(Req *r)
     cv.notify_all();
 }
 
+// Copies a finished chunk's results into the caller's buffers (stager
+// thread, without the lock); false on a HIP error.
+bool pm_server::copy_back(const Chunk &c)
+{
+    const Req *r = c.r;
+    const size_t p0 = c.p0, n = c.px, o4 = p0 * 4 * r->L;
+    bool ok = hipMemcpyAsync(r->models + p0 * 11, r->d_models + p0 * 11, n * 11 * sizeof(double),
+                             hipMemcpyDeviceToHost, xfer) == hipSuccess &&
+              hipMemcpyAsync(r->chi2 + p0, r->d_chi2 + p0, n * sizeof(float),
+                             hipMemcpyDeviceToHost, xfer) == hipSuccess &&
+              hipMemcpyAsync(r->iters + p0, r->d_iters + p0, n * sizeof(int),
+                             hipMemcpyDeviceToHost, xfer) == hipSuccess &&
+              (!r->accepted || hipMemcpyAsync(r->accepted + p0, r->d_acc + p0, n * sizeof(int),
+                                              hipMemcpyDeviceToHost, xfer) == hipSuccess) &&
+              (!r->fit || hipMemcpyAsync(r->fit + o4, r->d_fit + o4, n * 4 * r->L * sizeof(float),
+                                         hipMemcpyDeviceToHost, xfer) == hipSuccess) &&
+              hipStreamSynchronize(xfer) == hipSuccess;
+    return ok;
+}
+
 void pm_server::stager_loop()
 {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
-        if (!finished.empty()) {
-            Req *r = finished.front();
-            finished.pop_front();
+        if (!back.empty()) {
+            Chunk c = back.front();
+            back.pop_front();
+            const bool want = c.r->rc == PM_OK;
             lk.unlock();
-            finish_one(r);
+            const bool ok = !want || copy_back(c);
             lk.lock();
+            if (!ok) c.r->rc = PM_EDEVICE;
+            inflight--;
+            complete(c.r, c.px);
             continue;
         }
         if (!queued.empty() && admit_one(lk)) continue;
-        if (stop && queued.empty() && ready.empty() && finished.empty()) break;
+        if (stop && idle()) break;
         cv.wait(lk);
     }
+    cv.notify_all();
+}
+
+// Uploads and launches the next chunk of r on sl (dispatcher thread; the
+// lock is released for the upload and the launch: the scan's device buffers
+// stay put until all its chunks are back, and the slot is the dispatcher's).
+bool pm_server::launch(Slot &sl, Req *r, std::unique_lock<std::mutex> &lk)
+{
+    const int p0 = r->next_px, px = std::min(r->chunk, r->npix - p0);
+    const size_t o4 = (size_t)p0 * 4 * r->L, cube = (size_t)px * 4 * r->L * sizeof(float);
+    r->next_px += px;
+    if (r->next_px >= r->npix) ready.erase(std::find(ready.begin(), ready.end(), r));
+    inflight++;
+    const bool ok_before = r->rc == PM_OK;
+    lk.unlock();
+    hipEvent_t ev = nullptr;
+    if (!sl.spare.empty()) { ev = sl.spare.back(); sl.spare.pop_back(); }
+    else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
+    int rc = ev ? PM_OK : PM_EDEVICE;
+    if (rc == PM_OK && ok_before) {
+        if (hipMemcpyAsync(r->d_stokes + o4, r->stokes + o4, cube, hipMemcpyHostToDevice, sl.s) !=
+                hipSuccess ||
+            (r->slight && hipMemcpyAsync(r->d_slight + o4, r->slight + o4, cube,
+                                         hipMemcpyHostToDevice, sl.s) != hipSuccess))
+            rc = PM_EDEVICE;
+        if (rc == PM_OK)
+            rc = pm_invert_launch(r->ctx, r->d_stokes + o4, r->d_slight ? r->d_slight + o4 : nullptr,
+                                  px, r->d_models + (size_t)p0 * 11, r->d_chi2 + p0,
+                                  r->d_iters + p0, r->d_acc + p0, r->d_fit ? r->d_fit + o4 : nullptr,
+                                  sl.counters, sl.fallbacks, sl.s);
+        if (rc == PM_OK && hipEventRecord(ev, sl.s) != hipSuccess) rc = PM_EDEVICE;
+    }
+    lk.lock();
+    if (rc != PM_OK || r->rc != PM_OK) {
+        if (rc != PM_OK) r->rc = rc;
+        if (ev) sl.spare.push_back(ev);
+        back.push_back({r, p0, px, nullptr});      // completes without a copy
+        cv.notify_all();
+        return false;
+    }
+    sl.cur = {r, p0, px, ev};
+    sl.busy = true;
+    return true;
 }
 
 void pm_server::dispatcher_loop()
@@ -246,76 +299,35 @@ void pm_server::dispatcher_loop()
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
         bool busy = false;
-        // reap completed chunks
         for (auto &sl : slots) {
-            while (!sl.q.empty()) {
-                hipError_t e = hipEventQuery(sl.q.front().ev);
-                if (e == hipErrorNotReady) break;
-                Inflight f = sl.q.front();
-                sl.q.pop_front();
-                sl.spare.push_back(f.ev);
-                if (e != hipSuccess) f.r->rc = PM_EDEVICE;
-                f.r->done_px += f.px;
-                if (f.r->done_px == f.r->npix) {
-                    f.r->state = FINISHED;
-                    ready.erase(std::find(ready.begin(), ready.end(), f.r));
-                    finished.push_back(f.r);
-                    cv.notify_all();
-                }
+            if (sl.busy) {
+                hipError_t e = hipEventQuery(sl.cur.ev);
+                if (e == hipErrorNotReady) { busy = true; continue; }
+                if (e != hipSuccess) sl.cur.r->rc = PM_EDEVICE;
+                sl.spare.push_back(sl.cur.ev);
+                back.push_back(sl.cur);
+                sl.busy = false;
+                cv.notify_all();
             }
-            busy |= !sl.q.empty();
-        }
-        // feed the streams, least remaining work first
-        for (auto &sl : slots) {
-            while ((int)sl.q.size() < DEPTH) {
-                Req *best = nullptr;
-                double best_w = 0;
-                for (Req *r : ready) {
-                    if (r->next_px >= r->npix) continue;
-                    double w = (r->npix - r->next_px) * r->cost;
-                    if (!best || w < best_w) { best = r; best_w = w; }
-                }
-                if (!best) break;
-                Req *r = best;
-                const int p0 = r->next_px, px = std::min(r->chunk, r->npix - p0);
-                const size_t o4 = (size_t)p0 * 4 * r->L;
-                int rc = pm_invert_launch(r->ctx, r->d_stokes + o4, r->d_slight ? r->d_slight + o4 : nullptr,
-                                          px, r->d_models + (size_t)p0 * 11, r->d_chi2 + p0,
-                                          r->d_iters + p0, r->d_acc + p0,
-                                          r->d_fit ? r->d_fit + o4 : nullptr, sl.counters,
-                                          sl.fallbacks, sl.s);
-                hipEvent_t ev = nullptr;
-                if (!sl.spare.empty()) { ev = sl.spare.back(); sl.spare.pop_back(); }
-                else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) ev = nullptr;
-                if (rc != PM_OK || !ev || hipEventRecord(ev, sl.s) != hipSuccess) {
-                    if (rc == PM_OK) rc = PM_EDEVICE;
-                    r->rc = rc;
-                    if (ev) sl.spare.push_back(ev);
-                    // the chunk is counted as done so the scan completes
-                    r->next_px += px;
-                    r->done_px += px;
-                    if (r->done_px == r->npix) {
-                        r->state = FINISHED;
-                        ready.erase(std::find(ready.begin(), ready.end(), r));
-                        finished.push_back(r);
-                        cv.notify_all();
-                    }
-                    continue;
-                }
-                r->next_px += px;
-                sl.q.push_back({r, px, ev});
-                busy = true;
+            // next chunk: from the admitted scan with the least work left
+            Req *best = nullptr;
+            double best_w = 0;
+            for (Req *r : ready) {
+                double w = (r->npix - r->next_px) * r->cost;
+                if (!best || w < best_w) { best = r; best_w = w; }
             }
+            if (best && launch(sl, best, lk)) busy = true;
         }
-        if (stop && queued.empty() && ready.empty() && !busy) break;
+        if (stop && idle()) break;
         if (busy) {
             lk.unlock();
             std::this_thread::sleep_for(std::chrono::microseconds(50));
             lk.lock();
-        } else {
+        } else if (ready.empty()) {
             cv.wait(lk);
         }
     }
+    cv.notify_all();
 }
 
 extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
@@ -325,8 +337,8 @@ extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
     pm_server_opts o = {};
     if (opts) o = *opts;
     if (o.streams <= 0) o.streams = 3;
-    if (o.streams > 8) return PM_EINVAL;
-    if (o.chunk_px <= 0) o.chunk_px = 65536;
+    if (o.streams > 8 || o.chunk_px < 0) return PM_EINVAL;
+    if (o.chunk_px == 0) o.chunk_px = 65536;
     pm_server *s = new pm_server();
     if (hipGetDevice(&s->device) != hipSuccess) { delete s; return PM_EDEVICE; }
     if (o.hbm_budget == 0) {
@@ -336,6 +348,9 @@ extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
     }
     s->budget = o.hbm_budget;
     s->chunk_px = o.chunk_px;
+    // per-pixel cost of the reference workload: 30 samples, 9 parameters,
+    // a PSF of about 23 taps
+    s->ref_cost = 30 * (400.0 + 10.0 * 9 * 23) + 10.0 * 729;
     int rc = PM_OK;
     if (hipStreamCreateWithFlags(&s->xfer, hipStreamNonBlocking) != hipSuccess) rc = PM_EDEVICE;
     s->slots.resize(o.streams);
@@ -369,7 +384,6 @@ extern "C" void pm_server_destroy(pm_server *s)
     (void)hipSetDevice(s->device);
     for (auto &sl : s->slots) {
         if (sl.s) (void)hipStreamSynchronize(sl.s);
-        for (auto &f : sl.q) (void)hipEventDestroy(f.ev);
         for (auto ev : sl.spare) (void)hipEventDestroy(ev);
         if (sl.s) (void)hipStreamDestroy(sl.s);
         if (sl.counters) (void)hipFree(sl.counters);
@@ -401,9 +415,9 @@ extern "C" int pm_submit(pm_server *s, pm_ctx *ctx, const float *stokes, const f
     r->fit = fit;
     r->bytes = req_bytes(npix, r->L, slight != nullptr, fit != nullptr);
     if (r->bytes > s->budget) { delete r; return PM_ENOMEM; }
-    r->cost = (double)r->L * pm_ctx_nterms(ctx);
-    // chunks of about the same work whatever the sampling
-    r->chunk = std::max(4096, (int)((long long)s->chunk_px * 30 / std::max(r->L, 1)));
+    r->cost = pm_ctx_cost(ctx);
+    // chunks of about the work of chunk_px pixels of the reference workload
+    r->chunk = (int)std::max(1024.0, std::min(1e9, s->chunk_px * s->ref_cost / r->cost));
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
     r->id = s->next_id++;
