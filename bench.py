@@ -87,6 +87,81 @@ def cpu_baseline(per_core):
                        % (n, cores, per_core, slowest))
 
 
+def uneven_batch(args, torch, pmilos, precision):
+    """Three cubes in one launch: 512x512 px at 6 wavelengths (69 mA),
+    512x512 px at 30 (20 mA), 256x256 px of Fe I 6302.4936 A at 112 (21.5 mA)
+    with v_mac ~ U(0,2) km/s free (10 parameters); model seeds 41/42/43,
+    noise 1e-3 Ic.  Each sampling's kernel shape is picked by the autotuner
+    on its cube's first 32768 pixels; the batch is one HIP graph launch."""
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    specs = [(512 * 512, (FE6173, -172.5, 69, 172.5), False, 41),
+             (512 * 512, (FE6173, -290, 20, 290), False, 42),
+             (256 * 256, (6302.4936, -1193.25, 21.5, 1193.25), True, 43)]
+    items, keep, shapes = [], [], []
+    total = 0
+    for n, g, mac, seed in specs:
+        lam = pmilos.grid(*g)
+        L = len(lam)
+        guess, fix = GUESS.copy(), [1] * 9 + [0, 0]
+        if mac:
+            guess[9], fix = 0.5, [1] * 10 + [0]
+        ctx = pmilos.Context(g[0], TRIPLET, lam, psf=pmilos.psf_gaussian(60.0, lam), guess=guess,
+                             fix=fix, noise=1e-6, miter=50, precision=precision)
+        m = draw_models(n, seed)
+        if mac:
+            m[:, 9] = np.random.default_rng(seed + 1000).uniform(0, 2, n)
+        mt = torch.from_numpy(m).to(dev)
+        spec = torch.empty((n, 4, L), dtype=torch.float64, device=dev)
+        ctx.synth_ptrs(mt.data_ptr(), n, spec.data_ptr(), 0, st.cuda_stream)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed + 100)
+        ic = (mt[:, 7] + mt[:, 8]).view(-1, 1, 1)
+        cube = (spec + torch.randn(spec.shape, dtype=torch.float64, device=dev, generator=gen)
+                * 1e-3 * ic).to(torch.float32).contiguous()
+        del spec
+        o = [torch.empty((n, 11), dtype=torch.float64, device=dev),
+             torch.empty(n, dtype=torch.float32, device=dev),
+             torch.empty(n, dtype=torch.int32, device=dev),
+             torch.empty(n, dtype=torch.int32, device=dev)]
+        shapes.append({"nlambda": L, "npix": n, "nterms": ctx.NT,
+                       "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup":
+                           list(ctx.autotune_ptrs(cube.data_ptr(), min(n, 32768), st.cuda_stream))})
+        keep.append((ctx, cube, o))
+        items.append(dict(ctx=ctx, d_stokes=cube.data_ptr(), npix=n, d_models=o[0].data_ptr(),
+                          d_chi2=o[1].data_ptr(), d_iters=o[2].data_ptr(),
+                          d_accepted=o[3].data_ptr()))
+        total += n
+    graph = pmilos.Graph(items)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(args.steps):
+            fn()
+        e1.record(st)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / args.steps
+
+    ms_graph = timed(lambda: graph.launch(st.cuda_stream))
+
+    def one_by_one():
+        for it in items:
+            it["ctx"].invert_ptrs(it["d_stokes"], it["npix"], it["d_models"], it["d_chi2"],
+                                  it["d_iters"], it["d_accepted"], 0, st.cuda_stream)
+    ms_seq = timed(one_by_one)
+    iters = [float(k[2][2].float().mean().item()) for k in keep]
+    graph.close()
+    return {"workload": "uneven batch: 512^2 px x 6 lambda + 512^2 px x 30 lambda + 256^2 px x "
+                        "112 lambda (v_mac free), one graph launch",
+            "pixels": total, "ms_per_launch": round(ms_graph, 3),
+            "pixels_per_s": round(total / (ms_graph * 1e-3), 1),
+            "ms_one_by_one": round(ms_seq, 3), "cubes": shapes,
+            "mean_iters": [round(x, 3) for x in iters]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -102,6 +177,8 @@ def main():
     ap.add_argument("--serve-scans", type=int, default=32,
                     help="scans of the streaming request loop (0 = skip it)")
     ap.add_argument("--serve-util", type=float, default=0.5)
+    ap.add_argument("--no-batch", dest="batch", action="store_false",
+                    help="skip the uneven-batch (one graph launch) measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (--gpus 1); multi-GPU sharding "
@@ -215,6 +292,9 @@ def main():
             "p99_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
                                              [float("%.3g" % v) for v in np.percentile(d, 99, 0)])),
         }
+    if args.batch:
+        line["uneven_batch"] = {"fp64": uneven_batch(args, torch, pmilos, pmilos.PM_FP64),
+                                "mixed": uneven_batch(args, torch, pmilos, pmilos.PM_MIXED)}
     if args.serve_scans > 0:
         from pmilos import serving
         del stokes, out_m, out_c, out_i, out_a

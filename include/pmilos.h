@@ -136,6 +136,54 @@ int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_sligh
  * synchronises the context's last stream). */
 int pm_last_fallback_count(pm_ctx *ctx, long long *count);
 
+/* ------------------------------------------------------------ tuning and batches */
+
+/* Autotuner: times every inversion-kernel variant this context's sampling
+ * allows -- lanes per pixel, wavelength samples per lane, pixels per
+ * workgroup -- on npix pixels of d_stokes (device; d_slight as for
+ * pm_invert_sl_device) and keeps the fastest for this context's later
+ * inversions of that kind (with or without a stray-light cube).  chosen
+ * (nullable) = {lanes per pixel, samples per lane, pixels per workgroup}.
+ * Synchronises stream.  The pixels per workgroup do not change a pixel's
+ * arithmetic; the lane mapping changes the order of the sums over
+ * wavelength, so results of two variants agree to rounding (and the
+ * inversions to the stated parameter tolerances). */
+int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
+                    void *stream, int chosen[3]);
+
+/* The variant in use for the 9-row (general = 0) or general form: out as
+ * chosen above; returns the number of variants there are. */
+int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3]);
+
+/* Selects variant index (0 = the default mapping) of that form by hand. */
+int pm_ctx_set_variant(pm_ctx *ctx, int general, int index);
+
+/* One launch for several inversions (e.g. cubes of different samplings):
+ * each item is a pm_invert_sl_device call on device buffers.  They are
+ * captured into one HIP graph, one concurrent branch per item with its own
+ * work-queue counters, and pm_graph_launch runs the whole batch as one
+ * graph launch, as often as wanted (the buffers are fixed at creation).
+ * share_by_work > 0 caps each branch at its share of the estimated work of
+ * the batch (the branches then run side by side); 0 lets every branch fill
+ * the GPU (later branches take over as earlier ones drain). */
+typedef struct {
+    pm_ctx *ctx;
+    const float *d_stokes;
+    const float *d_slight;       /* nullable */
+    int npix;
+    double *d_models;
+    float *d_chi2;
+    int *d_iters;
+    int *d_accepted;             /* nullable */
+    float *d_fit;                /* nullable */
+} pm_batch_item;
+
+typedef struct pm_graph pm_graph;
+
+int  pm_graph_create(const pm_batch_item *items, int n, double share_by_work, pm_graph **out);
+int  pm_graph_launch(pm_graph *g, void *stream);
+void pm_graph_destroy(pm_graph *g);
+
 /* ------------------------------------------------------------ serving
  *
  * A server takes scans (Stokes cubes of any size, on any context of its

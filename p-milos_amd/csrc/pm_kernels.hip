@@ -13,6 +13,9 @@
 
 #include <math.h>
 #include <stdio.h>
+
+#include <algorithm>
+#include <vector>
 #include <stdlib.h>
 #include <string.h>
 
@@ -30,7 +33,7 @@ template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM
 
 // ------------------------------------------------------------ LDS layout
 
-template <typename T, int W, int NLPL, int NT, bool GEN = false>
+template <typename T, int W, int NLPL, int NT, bool GEN = false, int BLK = BlockOf<W>::value>
 struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
     static constexpr int CH = W >= 16 ? 16 : W;
@@ -51,7 +54,7 @@ struct Lds {
     // macroturbulence kernel and its v_mac derivative, both PSF-folded
     static constexpr int MAC_BYTES = GEN ? ((2 * W * NLPL * (int)sizeof(T) + 15) / 16 * 16) : 0;
     static constexpr int GROUP_BYTES = U_BYTES + OBS_BYTES + AB_BYTES + LM_BYTES + MAC_BYTES;
-    static constexpr int BLOCK = BlockOf<W>::value;
+    static constexpr int BLOCK = BLK;
     static constexpr int GPB = BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
     static_assert(BLOCK_BYTES <= 65536, "LDS per workgroup above 64 KiB");
@@ -447,11 +450,11 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // reference's run does not have are fixed rows here, zeroed like FijaACero
 // does, so alpha, beta and delta agree with the reference's NTERMS-sized
 // ones on the rows it has.  nfree uses the reference's NTERMS.
-template <typename T, int W, int NLPL, int NT, bool GEN>
-__global__ void __launch_bounds__(BlockOf<W>::value, (NLPL == 1 && !GEN ? 2 : 1))
+template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
+__global__ void __launch_bounds__(BLK, (NLPL == 1 && !GEN ? 2 : 1))
 pm_invert_kernel(InvArgs a)
 {
-    using LD = Lds<T, W, NLPL, NT, GEN>;
+    using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
     constexpr int NP = LD::NP;
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
     const PmConst *__restrict__ c = a.c;
@@ -810,6 +813,7 @@ struct pm_ctx {
     int device = 0;
     int ncu = 0;
     hipStream_t last_stream = nullptr;
+    int variant[2] = {0, 0};   // kernel variant of the 9-row and general forms
 };
 
 extern "C" const char *pm_strerror(int status)
@@ -1010,16 +1014,18 @@ extern "C" int pm_ctx_nterms(const pm_ctx *ctx) { return ctx ? ctx->h.NT : PM_EI
 
 // ------------------------------------------------------------ dispatch
 
-template <typename T, int W, int NLPL, int NT, bool GEN>
-static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st)
+// share: the fraction of the GPU's resident workgroups the launch may take
+// (1 = all of them).
+template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
+static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st, double share)
 {
-    using LD = Lds<T, W, NLPL, NT, GEN>;
-    auto kern = pm_invert_kernel<T, W, NLPL, NT, GEN>;
+    using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
+    auto kern = pm_invert_kernel<T, W, NLPL, NT, GEN, BLK>;
     int per_cu = 0;
     PM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, LD::BLOCK, 0));
     if (per_cu < 1) per_cu = 1;
     long long need = ((long long)a.npix + LD::GPB - 1) / LD::GPB;
-    long long grid = (long long)per_cu * ctx->ncu;
+    long long grid = (long long)(per_cu * ctx->ncu * (share > 0 && share < 1 ? share : 1.0) + 0.5);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, a.counters, a.fallbacks);
@@ -1047,7 +1053,8 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
 // Lanes per pixel for L wavelengths: the smallest power of two >= L up to a
 // full wavefront, then two samples per lane.  T = double (FP64 form) or float
 // (mixed form: spectra, RFs and convolution in FP32).  The general form
-// (NTERMS 10/11, macroturbulence, stray light) is the 11-row kernel.
+// (NTERMS 10/11, macroturbulence, stray light) is the 11-row kernel.  The
+// synthesis kernel always uses this mapping.
 #define PM_DISPATCH_LT(T, NT, G, L, FN, ...)                                  \
     ((L) <= 8 ? FN<T, 8, 1, NT, G>(__VA_ARGS__) :                             \
      (L) <= 16 ? FN<T, 16, 1, NT, G>(__VA_ARGS__) :                           \
@@ -1061,6 +1068,57 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
            : ((ctx)->precision == PM_MIXED                                    \
                   ? PM_DISPATCH_LT(float, 9, false, L, FN, __VA_ARGS__)       \
                   : PM_DISPATCH_LT(double, 9, false, L, FN, __VA_ARGS__)))
+
+// Inversion kernel variants the autotuner chooses from: lanes per pixel W,
+// samples per lane NLPL (W * NLPL >= L) and threads per workgroup (pixels
+// per workgroup = threads / W).  The first of each list is the default
+// mapping above.
+typedef int (*InvLaunch)(pm_ctx *, const InvArgs &, hipStream_t, double);
+struct InvVariant {
+    int W, NLPL, BLK;
+    InvLaunch fn;
+};
+#define PM_V(T, W, NLPL, NT, G, BLK) InvVariant{W, NLPL, BLK, &launch_invert<T, W, NLPL, NT, G, BLK>}
+
+template <typename T, int NT, bool G>
+static int variants_of(int L, InvVariant *v)
+{
+    int n = 0;
+    if (L <= 8) {
+        v[n++] = PM_V(T, 8, 1, NT, G, 128);
+    } else if (L <= 16) {
+        v[n++] = PM_V(T, 16, 1, NT, G, 256);
+        v[n++] = PM_V(T, 16, 1, NT, G, 128);
+        if constexpr (!G) v[n++] = PM_V(T, 8, 2, NT, G, 128);
+    } else if (L <= 32) {
+        v[n++] = PM_V(T, 32, 1, NT, G, 256);
+        v[n++] = PM_V(T, 32, 1, NT, G, 128);
+        if constexpr (!G) {
+            v[n++] = PM_V(T, 16, 2, NT, G, 256);
+            v[n++] = PM_V(T, 16, 2, NT, G, 128);
+        }
+    } else if (L <= 64) {
+        v[n++] = PM_V(T, 64, 1, NT, G, 256);
+        v[n++] = PM_V(T, 64, 1, NT, G, 128);
+        if constexpr (!G) v[n++] = PM_V(T, 32, 2, NT, G, 256);
+    } else {
+        v[n++] = PM_V(T, 64, 2, NT, G, 256);
+        v[n++] = PM_V(T, 64, 2, NT, G, 128);
+    }
+    return n;
+}
+
+constexpr int PM_MAXVAR = 8;
+
+static int variants(const pm_ctx *ctx, bool gen, InvVariant *v)
+{
+    const int L = ctx->h.L;
+    if (gen)
+        return ctx->precision == PM_MIXED ? variants_of<float, 11, true>(L, v)
+                                          : variants_of<double, 11, true>(L, v);
+    return ctx->precision == PM_MIXED ? variants_of<float, 9, false>(L, v)
+                                      : variants_of<double, 9, false>(L, v);
+}
 
 // The 9-row kernel covers NTERMS 9 without stray light; everything else
 // takes the general one.
@@ -1084,7 +1142,7 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
 int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
                      double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
                      float *d_fit, int *d_counters, unsigned long long *d_fallbacks,
-                     hipStream_t st)
+                     hipStream_t st, double share)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
@@ -1102,9 +1160,11 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
     a.counters = d_counters;
     a.fallbacks = d_fallbacks;
     a.slight = d_slight;
-    const int L = ctx->h.L;
     const bool gen = needs_general(ctx, d_slight != nullptr);
-    return PM_DISPATCH_L(ctx, gen, L, launch_invert, ctx, a, st);
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, gen, v);
+    const int k = ctx->variant[gen] < nv ? ctx->variant[gen] : 0;
+    return v[k].fn(ctx, a, st, share);
 }
 
 int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
@@ -1130,7 +1190,7 @@ extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const flo
     hipStream_t st = (hipStream_t)stream;
     ctx->last_stream = st;
     return pm_invert_launch(ctx, d_stokes, d_slight, npix, d_models, d_chi2, d_iters, d_accepted,
-                            d_fit, ctx->counters, ctx->fallbacks, st);
+                            d_fit, ctx->counters, ctx->fallbacks, st, 1.0);
 }
 
 extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
@@ -1148,6 +1208,185 @@ extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
     PM_HIP(hipStreamSynchronize(ctx->last_stream));
     PM_HIP(hipMemcpy(&v, ctx->fallbacks, sizeof(v), hipMemcpyDeviceToHost));
     *count = (long long)v;
+    return PM_OK;
+}
+
+// ------------------------------------------------------------ autotuner
+
+extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
+                               void *stream, int chosen[3])
+{
+    if (!ctx || !d_stokes || npix < 1) return PM_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    const bool gen = needs_general(ctx, d_slight != nullptr);
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, gen, v);
+    const size_t n = npix;
+    void *buf = nullptr;
+    if (hipMalloc(&buf, n * (11 * sizeof(double) + 3 * sizeof(int))) != hipSuccess)
+        return PM_ENOMEM;
+    double *dm = (double *)buf;
+    float *dc = (float *)(dm + 11 * n);
+    int *di = (int *)(dc + n), *da = di + n;
+    InvArgs a;
+    a.c = ctx->d;
+    a.stokes = d_stokes;
+    a.npix = npix;
+    a.slice = (npix + PM_NSLICE - 1) / PM_NSLICE;
+    a.models = dm;
+    a.chi2 = dc;
+    a.iters = di;
+    a.accepted = da;
+    a.fit = nullptr;
+    a.counters = ctx->counters;
+    a.fallbacks = ctx->fallbacks;
+    a.slight = d_slight;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = PM_OK;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = PM_EDEVICE;
+    int best = 0;
+    float best_ms = 0;
+    for (int k = 0; k < nv && rc == PM_OK; k++) {
+        float ms = 0;
+        rc = v[k].fn(ctx, a, st, 1.0);                       // first run: not timed
+        if (rc == PM_OK && hipEventRecord(e0, st) != hipSuccess) rc = PM_EDEVICE;
+        for (int rep = 0; rep < 2 && rc == PM_OK; rep++) rc = v[k].fn(ctx, a, st, 1.0);
+        if (rc == PM_OK && (hipEventRecord(e1, st) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = PM_EDEVICE;
+        if (rc == PM_OK && (k == 0 || ms < best_ms)) { best = k; best_ms = ms; }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(st);
+    (void)hipFree(buf);
+    if (rc != PM_OK) return rc;
+    ctx->variant[gen] = best;
+    if (chosen) { chosen[0] = v[best].W; chosen[1] = v[best].NLPL; chosen[2] = v[best].BLK / v[best].W; }
+    return PM_OK;
+}
+
+extern "C" int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3])
+{
+    if (!ctx || !out) return PM_EINVAL;
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, general != 0, v);
+    const int k = ctx->variant[general != 0] < nv ? ctx->variant[general != 0] : 0;
+    out[0] = v[k].W;
+    out[1] = v[k].NLPL;
+    out[2] = v[k].BLK / v[k].W;
+    return nv;
+}
+
+extern "C" int pm_ctx_set_variant(pm_ctx *ctx, int general, int index)
+{
+    if (!ctx) return PM_EINVAL;
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, general != 0, v);
+    if (index < 0 || index >= nv) return PM_EINVAL;
+    ctx->variant[general != 0] = index;
+    return PM_OK;
+}
+
+// ------------------------------------------------------------ one launch for a batch
+
+struct pm_graph {
+    int n = 0;
+    std::vector<int *> counters;
+    std::vector<unsigned long long *> fallbacks;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" void pm_graph_destroy(pm_graph *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    for (auto p : g->counters) if (p) (void)hipFree(p);
+    for (auto p : g->fallbacks) if (p) (void)hipFree(p);
+    delete g;
+}
+
+extern "C" int pm_graph_create(const pm_batch_item *items, int n, double share_by_work,
+                               pm_graph **out)
+{
+    if (!items || n < 1 || !out) return PM_EINVAL;
+    *out = nullptr;
+    for (int i = 0; i < n; i++) {
+        const pm_batch_item &b = items[i];
+        if (!b.ctx || b.npix < 1 || !b.d_stokes || !b.d_models || !b.d_chi2 || !b.d_iters)
+            return PM_EINVAL;
+        if (b.ctx->device != items[0].ctx->device) return PM_EINVAL;
+    }
+    pm_graph *g = new pm_graph();
+    g->n = n;
+    g->counters.assign(n, nullptr);
+    g->fallbacks.assign(n, nullptr);
+    for (int i = 0; i < n; i++)
+        if (hipMalloc(&g->counters[i], sizeof(int) * PM_NSLICE * PM_CSTRIDE) != hipSuccess ||
+            hipMalloc(&g->fallbacks[i], sizeof(unsigned long long)) != hipSuccess) {
+            pm_graph_destroy(g);
+            return PM_ENOMEM;
+        }
+    // each branch's share of the GPU: its estimated work (pixels x cost per
+    // pixel) over the batch's, when asked; else every branch may fill it
+    std::vector<double> share(n, 1.0);
+    if (share_by_work > 0) {
+        double tot = 0;
+        for (int i = 0; i < n; i++) tot += items[i].npix * pm_ctx_cost(items[i].ctx);
+        for (int i = 0; i < n; i++)
+            share[i] = std::max(0.02, items[i].npix * pm_ctx_cost(items[i].ctx) / tot);
+    }
+    std::vector<hipStream_t> br(n, nullptr);
+    std::vector<hipEvent_t> ev(n + 1, nullptr);
+    hipStream_t s0 = nullptr;
+    int rc = PM_OK;
+    if (hipStreamCreateWithFlags(&s0, hipStreamNonBlocking) != hipSuccess) rc = PM_EDEVICE;
+    for (int i = 0; i < n && rc == PM_OK; i++)
+        if (hipStreamCreateWithFlags(&br[i], hipStreamNonBlocking) != hipSuccess) rc = PM_EDEVICE;
+    for (int i = 0; i <= n && rc == PM_OK; i++)
+        if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) rc = PM_EDEVICE;
+    bool capturing = false;
+    if (rc == PM_OK) {
+        if (hipStreamBeginCapture(s0, hipStreamCaptureModeRelaxed) != hipSuccess) rc = PM_EDEVICE;
+        else capturing = true;
+    }
+    if (rc == PM_OK && hipEventRecord(ev[n], s0) != hipSuccess) rc = PM_EDEVICE;
+    for (int i = 0; i < n && rc == PM_OK; i++) {
+        const pm_batch_item &b = items[i];
+        if (hipStreamWaitEvent(br[i], ev[n], 0) != hipSuccess) { rc = PM_EDEVICE; break; }
+        rc = pm_invert_launch(b.ctx, b.d_stokes, b.d_slight, b.npix, b.d_models, b.d_chi2,
+                              b.d_iters, b.d_accepted, b.d_fit, g->counters[i], g->fallbacks[i],
+                              br[i], share[i]);
+        if (rc == PM_OK && (hipEventRecord(ev[i], br[i]) != hipSuccess ||
+                            hipStreamWaitEvent(s0, ev[i], 0) != hipSuccess))
+            rc = PM_EDEVICE;
+    }
+    if (capturing) {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(s0, &graph);
+        if (e != hipSuccess && rc == PM_OK) rc = PM_EDEVICE;
+        g->graph = graph;
+    }
+    if (rc == PM_OK && hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = PM_EDEVICE;
+    for (auto e : ev) if (e) (void)hipEventDestroy(e);
+    for (auto s : br) if (s) (void)hipStreamDestroy(s);
+    if (s0) (void)hipStreamDestroy(s0);
+    if (rc != PM_OK) {
+        pm_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return PM_OK;
+}
+
+extern "C" int pm_graph_launch(pm_graph *g, void *stream)
+{
+    if (!g || !g->exec) return PM_EINVAL;
+    PM_HIP(hipGraphLaunch(g->exec, (hipStream_t)stream));
     return PM_OK;
 }
 

@@ -277,7 +277,7 @@ bool pm_server::launch(Slot &sl, Req *r, std::unique_lock<std::mutex> &lk)
             rc = pm_invert_launch(r->ctx, r->d_stokes + o4, r->d_slight ? r->d_slight + o4 : nullptr,
                                   px, r->d_models + (size_t)p0 * 11, r->d_chi2 + p0,
                                   r->d_iters + p0, r->d_acc + p0, r->d_fit ? r->d_fit + o4 : nullptr,
-                                  sl.counters, sl.fallbacks, sl.s);
+                                  sl.counters, sl.fallbacks, sl.s, 1.0);
         if (rc == PM_OK && hipEventRecord(ev, sl.s) != hipSuccess) rc = PM_EDEVICE;
     }
     lk.lock();

@@ -39,6 +39,12 @@ class Config(C.Structure):
                 ("slight", C.POINTER(C.c_float))]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("ctx", C.c_void_p), ("d_stokes", C.c_void_p), ("d_slight", C.c_void_p),
+                ("npix", C.c_int), ("d_models", C.c_void_p), ("d_chi2", C.c_void_p),
+                ("d_iters", C.c_void_p), ("d_accepted", C.c_void_p), ("d_fit", C.c_void_p)]
+
+
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget", C.c_size_t), ("streams", C.c_int), ("chunk_px", C.c_int)]
 
@@ -67,6 +73,13 @@ def lib():
         L.pm_invert_sl.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp]
         L.pm_invert_sl_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.pm_last_fallback_count.argtypes = [vp, C.POINTER(C.c_longlong)]
+        L.pm_ctx_autotune.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_int * 3)]
+        L.pm_ctx_variant.argtypes = [vp, C.c_int, C.POINTER(C.c_int * 3)]
+        L.pm_ctx_set_variant.argtypes = [vp, C.c_int, C.c_int]
+        L.pm_graph_create.argtypes = [C.POINTER(BatchItem), C.c_int, C.c_double, C.POINTER(vp)]
+        L.pm_graph_launch.argtypes = [vp, vp]
+        L.pm_graph_destroy.argtypes = [vp]
+        L.pm_graph_destroy.restype = None
         L.pm_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
         L.pm_server_destroy.argtypes = [vp]
         L.pm_server_destroy.restype = None
@@ -189,6 +202,25 @@ class Context:
         _check(lib().pm_synth_device(self.h, d_models, n, d_spectra, d_rf or None,
                                      stream or None))
 
+    def autotune_ptrs(self, d_stokes, npix, stream=0, d_slight=0):
+        """Times the kernel variants on npix pixels at device address
+        d_stokes and keeps the fastest: (lanes per pixel, samples per lane,
+        pixels per workgroup)."""
+        out = (C.c_int * 3)()
+        _check(lib().pm_ctx_autotune(self.h, d_stokes, d_slight or None, npix, stream or None,
+                                     C.byref(out)))
+        return tuple(out)
+
+    def variant(self, general=False):
+        """((lanes per pixel, samples per lane, pixels per workgroup), number
+        of variants) of the form in use."""
+        out = (C.c_int * 3)()
+        n = _check(lib().pm_ctx_variant(self.h, int(general), C.byref(out)))
+        return tuple(out), n
+
+    def set_variant(self, index, general=False):
+        _check(lib().pm_ctx_set_variant(self.h, int(general), index))
+
     def fallback_count(self):
         v = C.c_longlong(0)
         _check(lib().pm_last_fallback_count(self.h, C.byref(v)))
@@ -239,6 +271,44 @@ class Server:
             lib().pm_server_destroy(self.h)
             self.h = None
             self._pending.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Graph:
+    """Several inversions on device buffers captured as one HIP graph
+    (pm_graph): launch() runs them all with one graph launch.  items: dicts
+    with ctx, d_stokes, npix, d_models, d_chi2, d_iters and optionally
+    d_slight, d_accepted, d_fit (device addresses)."""
+
+    def __init__(self, items, share_by_work=0.0):
+        arr = (BatchItem * len(items))()
+        self._ctxs = [it["ctx"] for it in items]
+        for a, it in zip(arr, items):
+            a.ctx = it["ctx"].h.value
+            a.d_stokes = it["d_stokes"]
+            a.d_slight = it.get("d_slight") or None
+            a.npix = it["npix"]
+            a.d_models = it["d_models"]
+            a.d_chi2 = it["d_chi2"]
+            a.d_iters = it["d_iters"]
+            a.d_accepted = it.get("d_accepted") or None
+            a.d_fit = it.get("d_fit") or None
+        h = C.c_void_p()
+        _check(lib().pm_graph_create(arr, len(items), share_by_work, C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=0):
+        _check(lib().pm_graph_launch(self.h, stream or None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().pm_graph_destroy(self.h)
+            self.h = None
 
     def __del__(self):
         try:

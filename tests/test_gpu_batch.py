@@ -1,0 +1,136 @@
+"""GPU tests of the kernel variants, the autotuner and the one-launch batch
+(pm_ctx_autotune, pm_ctx_set_variant, pm_graph_*).  Every variant of a
+sampling inverts to the oracle's results within the FP64 tolerances; a
+graph of several inversions gives, on every replay, the bits the same
+inversions give one by one."""
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+import oracle as O
+import pmilos
+from fixtures import FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models, oracle_cube
+
+pytestmark = pytest.mark.gpu
+
+GRIDS = {6: (FE6173, -172.5, 69, 172.5), 14: (FE6173, -260, 40, 260),
+         30: (FE6173, -290, 20, 290), 49: (FE6173, -600, 25, 600),
+         112: (FE6302, -1193.25, 21.5, 1193.25)}
+TOL64 = np.array([1e-3, 1e-2, 1e-6, 1e-8, 1e-6, 1e-4, 1e-4, 1e-6, 1e-6])
+# wider samplings and few samples part a few LM paths at rounding level
+TOL_WIDE = np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 5e-3, 5e-3])
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _pair(L, mac=False, precision=pmilos.PM_FP64):
+    g = GRIDS[L]
+    lam = O.grid(*g)
+    G = O.gauss_psf(60.0, lam)
+    guess, fix = GUESS.copy(), [1] * 9 + [0, 0]
+    if mac:
+        guess[9], fix = 0.5, [1] * 10 + [0]
+    s = O.make_setup(g[0], TRIPLET, lam, psf=G, fix=fix, guess_mac=guess[9])
+    ctx = pmilos.Context(g[0], TRIPLET, lam, psf=G, guess=guess, fix=fix, precision=precision)
+    return s, ctx, guess
+
+
+@pytest.mark.parametrize("L", [14, 30, 49, 112])
+def test_every_variant_matches_oracle(L):
+    s, ctx, guess = _pair(L)
+    truths = draw_models(384, seed=700 + L)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=701 + L)
+    o = O.invert(s, cube, guess)
+    _, nv = ctx.variant()
+    tol, frac = (TOL64, 0.995) if L == 30 else (TOL_WIDE, 0.99)
+    for k in range(nv):
+        ctx.set_variant(k)
+        shape, _ = ctx.variant()
+        say("case variant L=%d index=%d shape=%s" % (L, k, shape))
+        g = ctx.invert(cube)
+        ok = np.abs(g["model"][:, :9] - o["model"][:, :9]) <= tol
+        say("  within tolerance %s" % np.round(ok.mean(axis=0), 4))
+        assert np.all(ok.mean(axis=0) >= frac)
+
+
+def test_workgroup_size_does_not_change_bits():
+    """Variants that differ only in pixels per workgroup give equal bits."""
+    s, ctx, guess = _pair(30)
+    truths = draw_models(2000, seed=710)
+    cube = add_noise(ctx.synth(truths), truths, 1e-3, seed=711)
+    ctx.set_variant(0)
+    a = ctx.invert(cube)
+    ctx.set_variant(1)
+    assert ctx.variant()[0][:2] == (32, 1)
+    b = ctx.invert(cube)
+    for k in ("model", "chi2", "iters", "accepted"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_autotune_picks_a_variant_and_keeps_results():
+    say("case autotune L=30")
+    dev = torch.device("cuda:0")
+    s, ctx, guess = _pair(30)
+    truths = draw_models(16384, seed=720)
+    cube = add_noise(ctx.synth(truths), truths, 1e-3, seed=721)
+    d = torch.from_numpy(cube).to(dev)
+    chosen = ctx.autotune_ptrs(d.data_ptr(), 16384, torch.cuda.current_stream().cuda_stream)
+    say("  chosen %s" % (chosen,))
+    assert chosen == ctx.variant()[0]
+    assert chosen[0] * chosen[1] >= 30 and chosen[2] >= 1
+    o = O.invert(s, cube[:512], guess)
+    g = ctx.invert(cube[:512])
+    ok = np.abs(g["model"][:, :9] - o["model"][:, :9]) <= TOL64
+    assert np.all(ok.mean(axis=0) >= 0.995)
+
+
+def _dev_outputs(n, L, dev):
+    return dict(m=torch.empty((n, 11), dtype=torch.float64, device=dev),
+                c=torch.empty(n, dtype=torch.float32, device=dev),
+                i=torch.empty(n, dtype=torch.int32, device=dev),
+                a=torch.empty(n, dtype=torch.int32, device=dev),
+                f=torch.empty((n, 4, L), dtype=torch.float32, device=dev))
+
+
+@pytest.mark.parametrize("share", [0.0, 1.0])
+def test_graph_batch_equals_separate_inversions(share):
+    say("case graph batch share=%s" % share)
+    dev = torch.device("cuda:0")
+    st = torch.cuda.current_stream()
+    specs = [(6, False, 3000, 730), (30, False, 4000, 731), (112, True, 600, 732)]
+    items, eager, outs = [], [], []
+    for L, mac, n, seed in specs:
+        _, ctx, _ = _pair(L, mac)
+        m = draw_models(n, seed, mac_range=mac)
+        cube = add_noise(ctx.synth(m), m, 1e-3, seed + 1)
+        eager.append(ctx.invert(cube, want_fit=True))
+        d = torch.from_numpy(cube).to(dev)
+        o = _dev_outputs(n, L, dev)
+        outs.append((ctx, d, o))
+        items.append(dict(ctx=ctx, d_stokes=d.data_ptr(), npix=n, d_models=o["m"].data_ptr(),
+                          d_chi2=o["c"].data_ptr(), d_iters=o["i"].data_ptr(),
+                          d_accepted=o["a"].data_ptr(), d_fit=o["f"].data_ptr()))
+    g = pmilos.Graph(items, share_by_work=share)
+    try:
+        for rep in range(3):
+            for _, _, o in outs:
+                for t in o.values():
+                    t.fill_(-1)
+            torch.cuda.synchronize()
+            g.launch(st.cuda_stream)
+            torch.cuda.synchronize()
+            for (ctx, d, o), e in zip(outs, eager):
+                assert np.array_equal(o["m"].cpu().numpy(), e["model"]), rep
+                assert np.array_equal(o["c"].cpu().numpy(), e["chi2"]), rep
+                assert np.array_equal(o["i"].cpu().numpy(), e["iters"]), rep
+                assert np.array_equal(o["a"].cpu().numpy(), e["accepted"]), rep
+                assert np.array_equal(o["f"].cpu().numpy(), e["fit"]), rep
+            # eager work between replays
+            _ = (outs[0][1] * 2).sum().item()
+    finally:
+        g.close()
