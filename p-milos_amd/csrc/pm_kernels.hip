@@ -479,7 +479,8 @@ pm_invert_kernel(InvArgs a)
     double flambda = 0, ochisqr = 0, chisqr0 = 0, factor = 1.0;
     int it = 0, nacc = 0, nfree = 1, pix = -1;
     bool iterating = false;
-    float obs[4][NLPL];
+    // the observed samples stay in LDS (sobs) for the pixel's whole LM loop
+    // and are read there where needed, not carried in registers
     T S[4][NLPL];
     Samp<T> sp[NLPL];
     Geom<T> g;
@@ -511,7 +512,6 @@ pm_invert_kernel(InvArgs a)
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     float v = lam < L ? src[s * L + lam] : 0.0f;
-                    obs[s][k] = v;
                     if (lam < L) {
                         sobs[s * L + lam] = v;
                         ng += (v <= -1.0f);                 // milosUtils.c:484-492
@@ -599,8 +599,9 @@ pm_invert_kernel(InvArgs a)
             if (lam < L) {
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    double isg = obs[s][k] <= -1.0f ? 1e-21 : c->inv_noise;
-                    double d = (double)S[s][k] - (double)obs[s][k];
+                    const float o = sobs[s * L + lam];
+                    double isg = o <= -1.0f ? 1e-21 : c->inv_noise;
+                    double d = (double)S[s][k] - (double)o;
                     part += (((d * d) * c->weight[s]) * isg);
                 }
             }
@@ -656,12 +657,14 @@ pm_invert_kernel(InvArgs a)
             A wsig[NLPL][4], rs[NLPL][4];
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
+                const int lam = t + k * W;
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
-                    double isg = obs[s][k] <= -1.0f ? 1e-21 : c->inv_noise;
+                    const float o = lam < L ? sobs[s * L + lam] : 0.0f;
+                    double isg = o <= -1.0f ? 1e-21 : c->inv_noise;
                     double w = c->weight[s];
                     wsig[k][s] = A(w * isg);
-                    rs[k][s] = A((w * ((double)S[s][k] - (double)obs[s][k])) * isg);
+                    rs[k][s] = A((w * ((double)S[s][k] - (double)o)) * isg);
                 }
             }
             // The NP + NT sums are reduced CH at a time through LDS: every lane
@@ -1050,35 +1053,24 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
     return PM_OK;
 }
 
-// Lanes per pixel for L wavelengths: the smallest power of two >= L up to a
-// full wavefront, then two samples per lane.  T = double (FP64 form) or float
-// (mixed form: spectra, RFs and convolution in FP32).  The general form
-// (NTERMS 10/11, macroturbulence, stray light) is the 11-row kernel.  The
-// synthesis kernel always uses this mapping.
-#define PM_DISPATCH_LT(T, NT, G, L, FN, ...)                                  \
-    ((L) <= 8 ? FN<T, 8, 1, NT, G>(__VA_ARGS__) :                             \
-     (L) <= 16 ? FN<T, 16, 1, NT, G>(__VA_ARGS__) :                           \
-     (L) <= 32 ? FN<T, 32, 1, NT, G>(__VA_ARGS__) :                           \
-     (L) <= 64 ? FN<T, 64, 1, NT, G>(__VA_ARGS__) :                           \
-                 FN<T, 64, 2, NT, G>(__VA_ARGS__))
-#define PM_DISPATCH_L(ctx, gen, L, FN, ...)                                   \
-    ((gen) ? ((ctx)->precision == PM_MIXED                                    \
-                  ? PM_DISPATCH_LT(float, 11, true, L, FN, __VA_ARGS__)       \
-                  : PM_DISPATCH_LT(double, 11, true, L, FN, __VA_ARGS__))     \
-           : ((ctx)->precision == PM_MIXED                                    \
-                  ? PM_DISPATCH_LT(float, 9, false, L, FN, __VA_ARGS__)       \
-                  : PM_DISPATCH_LT(double, 9, false, L, FN, __VA_ARGS__)))
-
-// Inversion kernel variants the autotuner chooses from: lanes per pixel W,
-// samples per lane NLPL (W * NLPL >= L) and threads per workgroup (pixels
-// per workgroup = threads / W).  The first of each list is the default
-// mapping above.
+// Kernel variants the autotuner chooses from: lanes per pixel W, samples
+// per lane NLPL (W * NLPL >= L) and threads per workgroup (pixels per
+// workgroup = threads / W).  The first of each list is the default mapping:
+// the smallest power of two >= L lanes up to a full wavefront, then two
+// samples per lane.  T = double (FP64 form) or float (mixed form: spectra,
+// RFs and convolution in FP32); the general form (NTERMS 10/11,
+// macroturbulence, stray light) is the 11-row kernel.  The synthesis kernel
+// follows the selected variant's lane mapping.
 typedef int (*InvLaunch)(pm_ctx *, const InvArgs &, hipStream_t, double);
+typedef int (*SynLaunch)(pm_ctx *, const double *, int, double *, double *, hipStream_t);
 struct InvVariant {
     int W, NLPL, BLK;
     InvLaunch fn;
+    SynLaunch sfn;         // synthesis with the same lane mapping
 };
-#define PM_V(T, W, NLPL, NT, G, BLK) InvVariant{W, NLPL, BLK, &launch_invert<T, W, NLPL, NT, G, BLK>}
+#define PM_V(T, W, NLPL, NT, G, BLK)                                          \
+    InvVariant{W, NLPL, BLK, &launch_invert<T, W, NLPL, NT, G, BLK>,          \
+               &launch_synth<T, W, NLPL, NT, G>}
 
 template <typename T, int NT, bool G>
 static int variants_of(int L, InvVariant *v)
@@ -1134,9 +1126,11 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     if (n == 0) return PM_OK;
     hipStream_t st = (hipStream_t)stream;
     ctx->last_stream = st;
-    const int L = ctx->h.L;
     const bool gen = needs_general(ctx, false);
-    return PM_DISPATCH_L(ctx, gen, L, launch_synth, ctx, d_models, n, d_spectra, d_rf, st);
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, gen, v);
+    const int k = ctx->variant[gen] < nv ? ctx->variant[gen] : 0;
+    return v[k].sfn(ctx, d_models, n, d_spectra, d_rf, st);
 }
 
 int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,

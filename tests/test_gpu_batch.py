@@ -40,6 +40,38 @@ def _pair(L, mac=False, precision=pmilos.PM_FP64):
     return s, ctx, guess
 
 
+def _err_rel(a, b, axis):
+    scale = np.maximum(np.abs(b).max(axis=axis, keepdims=True), 1e-5)
+    return (np.abs(a - b) / scale).max()
+
+
+@pytest.mark.parametrize("L", [6, 14, 30, 49, 112])
+@pytest.mark.parametrize("precision", [pmilos.PM_FP64, pmilos.PM_MIXED])
+def test_every_variant_synthesis_matches_oracle(L, precision):
+    """The synthesis kernel follows the selected lane mapping: spectra and
+    response functions of every variant against the oracle."""
+    s, ctx, guess = _pair(L, precision=precision)
+    models = draw_models(24, seed=740 + L)
+    ref = [O.synth(s, m, rf=True) for m in models]
+    _, nv = ctx.variant()
+    for k in range(nv):
+        ctx.set_variant(k)
+        shape, _ = ctx.variant()
+        say("case variant synth L=%d precision=%d index=%d shape=%s" % (L, precision, k, shape))
+        sp, rf = ctx.synth(models, rf=True)
+        e_rf = max(_err_rel(rf[i], r[1], (0, 2)) for i, r in enumerate(ref))
+        if precision == pmilos.PM_FP64:
+            e_sp = max(_err_rel(sp[i], r[0], -1) for i, r in enumerate(ref))
+            say("  spectra %.3g  rf %.3g" % (e_sp, e_rf))
+            assert e_sp < 1e-10 and e_rf < 1e-10
+        else:
+            # FP32 spectra: absolute, against the intensity scale
+            e_sp = max(np.abs(sp[i] - r[0]).max() / np.abs(r[0][0]).max()
+                       for i, r in enumerate(ref))
+            say("  spectra %.3g  rf %.3g" % (e_sp, e_rf))
+            assert e_sp < 1e-6 and e_rf < 2e-3
+
+
 @pytest.mark.parametrize("L", [14, 30, 49, 112])
 def test_every_variant_matches_oracle(L):
     s, ctx, guess = _pair(L)
