@@ -36,6 +36,13 @@ TRIPLET = (2, 1, 1.0, 2, 2, 0.0)
 GUESS = np.array([20, 1100, 0.2, 0.03, 0.05, 120, 150, 0.35, 0.5, 0, 1.0])
 PEAK_FP64_TFLOPS = 78.6
 PEAK_FP32_TFLOPS = 157.3
+# HBM bytes per pixel of one pm_invert_kernel launch on this workload, from
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE runs of this script at
+# 1048576 pixels (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors
+# a plain 512 MiB streaming kernel reads in the same runs); the summaries
+# are profiles/pm_invert_kernel_l30_1048576px_hbm.md
+TRAFFIC_B_PER_PX = {"fp64": 591.45, "mixed": 594.91}
+ALGO_B_PER_PX = 4 * 30 * 4 + 11 * 8 + 3 * 4     # Stokes in; model, chi2, iters, accepted out
 
 
 def draw_models(n, seed):
@@ -177,6 +184,8 @@ def main():
     ap.add_argument("--serve-scans", type=int, default=32,
                     help="scans of the streaming request loop (0 = skip it)")
     ap.add_argument("--serve-util", type=float, default=0.5)
+    ap.add_argument("--no-autotune", dest="autotune", action="store_false",
+                    help="keep the default kernel shape instead of the autotuner's")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the uneven-batch (one graph launch) measurement")
     args = ap.parse_args()
@@ -226,6 +235,10 @@ def main():
     torch.cuda.synchronize()
 
     def run_form(c, peak):
+        # kernel shape by the autotuner, on the first 65536 pixels (untimed)
+        shape = c.autotune_ptrs(stokes.data_ptr(), min(npix, 65536), sptr) if args.autotune \
+            else c.variant()[0]
+
         def step():
             c.invert_ptrs(stokes.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
                           out_i.data_ptr(), out_a.data_ptr(), 0, sptr)
@@ -247,6 +260,7 @@ def main():
         fl = float(np.sum(flops.pixel_flops(iters, acc, L=L, NT=9)))
         achieved = fl / (kern_ms * 1e-3) / 1e12
         return dict(value=npix * args.steps / wall, ms_per_step=wall * 1000.0 / args.steps,
+                    shape=list(shape),
                     kernel_ms=kern_ms, achieved=achieved, frac=achieved / peak,
                     mean_iters=float(iters.mean()), mean_accepted=float(acc.mean()),
                     fallbacks=c.fallback_count(), models=out_m.cpu().numpy())
@@ -263,13 +277,18 @@ def main():
         "data": "synthetic (library FP64 synthesis of seeded models + seeded noise)",
         "config": {"workload": "1024x1024 px Fe I 6173.34 A, 30 lambda @ 20 mA, PSF 60 mA, "
                                "sigma 1e-3 Ic, 9 free, <=50 LM it, plain FP64 form",
-                   "npix": npix, "nlambda": L, "nterms": 9, "form": "fp64"},
+                   "npix": npix, "nlambda": L, "nterms": 9, "form": "fp64",
+                   "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup": r64["shape"]},
         "roofline": {"bound": "mfma", "achieved": round(r64["achieved"], 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": round(r64["frac"], 4), "traffic": None,
+                     "frac": round(r64["frac"], 4),
+                     "traffic": round(TRAFFIC_B_PER_PX["fp64"] * npix),
                      "note": "FP64 vector ALU work (the MI355X FP64 vector and matrix "
                              "peaks are both 78.6 TFLOP/s); achieved = algorithmic "
-                             "FLOPs of the pass / pm_invert_kernel time"},
+                             "FLOPs of the pass / pm_invert_kernel time; traffic = HBM "
+                             "bytes per launch from the FETCH_SIZE/WRITE_SIZE counters, "
+                             "%.0f B/pixel against %d algorithmic" % (TRAFFIC_B_PER_PX["fp64"],
+                                                                     ALGO_B_PER_PX)},
         "kernel_ms": round(r64["kernel_ms"], 3),
         "mean_iters": round(r64["mean_iters"], 3),
         "mean_accepted": round(r64["mean_accepted"], 3),
@@ -284,8 +303,10 @@ def main():
             "value": round(rmx["value"], 1), "unit": "pixels/s", "dtype": "f32+f64",
             "ms_per_step": round(rmx["ms_per_step"], 3), "kernel_ms": round(rmx["kernel_ms"], 3),
             "speedup_vs_fp64": round(rmx["value"] / r64["value"], 3),
+            "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup": rmx["shape"],
             "roofline": {"achieved": round(rmx["achieved"], 3), "peak": PEAK_FP32_TFLOPS,
-                         "unit": "TFLOP/s", "frac": round(rmx["frac"], 4)},
+                         "unit": "TFLOP/s", "frac": round(rmx["frac"], 4),
+                         "traffic": round(TRAFFIC_B_PER_PX["mixed"] * npix)},
             "mean_iters": round(rmx["mean_iters"], 3),
             "median_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
                                                 [float("%.3g" % v) for v in np.median(d, 0)])),
