@@ -77,6 +77,10 @@ typedef struct {
     const float *slight;         /* [4][L] stray-light profile (the .per file of
                                     readPerStrayLightFile, utilsFits.c:835-877),
                                     or NULL */
+    int no_classical;            /* 1: start every pixel from the guess itself,
+                                    as lm_mils on a .per profile (milos.c:741-775);
+                                    0: classical estimates and NaN guards first,
+                                    as the FITS pixel loop (milos.c:958-983) */
 } pm_config;
 
 const char *pm_strerror(int status);
@@ -131,6 +135,14 @@ int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_sligh
                         double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
                         float *d_fit, void *stream);
 
+/* Classical estimates only (NumberOfCycles < 0, milos.c:399-520): per
+ * pixel the context's guess with B, vlos, gamma, phi from
+ * estimacionesClasicas(forInitialUse = 0) and B_los in the S0 slot
+ * (milosUtils.c:449-450); models [npix][11]. */
+int pm_classical(pm_ctx *ctx, const float *stokes, int npix, double *models);
+int pm_classical_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
+                        void *stream);
+
 /* Number of pixels whose 9x9/10x10/11x11 solve needed the eigen
  * pseudo-inverse fallback in the last inversion on this context (diagnostic;
  * synchronises the context's last stream). */
@@ -150,6 +162,10 @@ int pm_last_fallback_count(pm_ctx *ctx, long long *count);
  * inversions to the stated parameter tolerances). */
 int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
                     void *stream, int chosen[3]);
+
+/* The same on host buffers (copied to the device for the trials). */
+int pm_ctx_autotune_host(pm_ctx *ctx, const float *stokes, const float *slight, int npix,
+                         int chosen[3]);
 
 /* The variant in use for the 9-row (general = 0) or general form: out as
  * chosen above; returns the number of variants there are. */

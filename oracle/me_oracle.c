@@ -800,8 +800,11 @@ void orc_fija_cero(const orc_setup_t *s, double *rf)
 /* Per pixel                                                                 */
 /* ------------------------------------------------------------------------ */
 
-/* Classical estimates (forInitialUse = 1), milosUtils.c:287-452. */
-void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR])
+/* Classical estimates, milosUtils.c:287-452.  for_init = forInitialUse:
+ * 1 for the inversion's starting point, 0 for the estimates-only mode
+ * (no 85-95 degree nudge of B's inclination; B_los stored in S0). */
+void orc_classical_ex(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],
+                      int for_init)
 {
     const int n = s->nlambda;
     const double l0 = s->lambda0;
@@ -854,8 +857,10 @@ void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_
     double tan_gamma = fabs(sqrt(x / y));
     double gamma_rad = atan(tan_gamma);
     double gamma = gamma_rad * (180 / ORC_PI);
-    if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (ORC_PI / 180);   /* :395-402 */
-    if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (ORC_PI / 180);
+    if (for_init) {                                                  /* :395-402 */
+        if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (ORC_PI / 180);
+        if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (ORC_PI / 180);
+    }
     if (Blos < 0) gamma = (180) - gamma;                            /* :405-406 */
 
     double su = 0.0, sq = 0.0;                                      /* :412-420 */
@@ -878,6 +883,12 @@ void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_
     model[ORC_VLOS] = Vlos;
     model[ORC_GM] = gamma;
     model[ORC_AZ] = phi;
+    if (!for_init) model[ORC_S0] = Blos;                            /* :449-450 */
+}
+
+void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR])
+{
+    orc_classical_ex(s, spectro, model, 1);
 }
 
 /* lm_mils, milosUtils.c:466-635. */

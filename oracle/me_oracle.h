@@ -100,6 +100,9 @@ void   orc_fija_cero(const orc_setup_t *s, double *rf);
 /* ---- per-pixel ---- */
 void orc_classical(const orc_setup_t *s, const float *spectro,
                    double model[ORC_NPAR]);
+/* for_init = 0: the estimates-only mode (B_los in S0, no inclination nudge) */
+void orc_classical_ex(const orc_setup_t *s, const float *spectro,
+                      double model[ORC_NPAR], int for_init);
 /* lm_mils: model is in/out.  fit (4L, nullable) receives the last trial
  * synthesis.  Returns 1, or -1 when nfree == 0 (milosUtils.c:509-512). */
 int  orc_lm(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],

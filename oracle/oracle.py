@@ -76,6 +76,7 @@ def lib():
         L.orc_check.argtypes = [_dp]
         L.orc_fija_cero.argtypes = [C.POINTER(Setup), _dp]
         L.orc_classical.argtypes = [C.POINTER(Setup), _fp, _dp]
+        L.orc_classical_ex.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_int]
         L.orc_lm.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_void_p,
                              C.POINTER(C.c_float), C.POINTER(C.c_int),
                              C.POINTER(C.c_int)]
@@ -195,9 +196,10 @@ def synth(s, model, rf=False):
     return spec.reshape(4, L)
 
 
-def classical(s, spectro, model):
+def classical(s, spectro, model, for_init=True):
     m = np.array(model, dtype=np.float64)
-    lib().orc_classical(C.byref(s), np.ascontiguousarray(spectro, dtype=np.float32).ravel(), m)
+    lib().orc_classical_ex(C.byref(s), np.ascontiguousarray(spectro, dtype=np.float32).ravel(),
+                           m, int(for_init))
     return m
 
 

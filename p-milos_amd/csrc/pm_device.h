@@ -46,6 +46,9 @@ struct PmConst {
     // (mil_sinrf.c:405-412), when use_slight
     int use_slight;
     float slight[4 * PM_LMAX];
+    // start every pixel from the guess itself, without classical estimates
+    // (lm_mils on a .per profile, milos.c:741-775)
+    int no_classical;
 };
 
 // ------------------------------------------------------------ group helpers
@@ -750,14 +753,19 @@ __device__ __noinline__ void jacobi_pinv(double *ws, int t)
 // ------------------------------------------------------------ classical estimates
 
 struct Classic {
-    double B, vlos, gm, az;
+    double B, vlos, gm, az, blos;
 };
 
-// estimacionesClasicas(forInitialUse = 1), milosUtils.c:287-452, on the
-// observed profile staged in LDS (each lane runs it redundantly, in the
-// reference's summation order), followed by the NaN guards of the pixel loop.
-// Not inlined: it runs once per pixel.
-__device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const float *spec)
+// estimacionesClasicas, milosUtils.c:287-452, on one observed profile (in
+// LDS when each lane of a pixel group runs it redundantly, in global memory
+// for the estimates-only pass), in the reference's summation order.  With
+// init (forInitialUse = 1, the inversion's starting point) the 85-95 degree
+// inclination nudge applies to B and the pixel loop's NaN guards follow
+// (milos.c:976-983); without it (the estimates-only mode, milos.c:493) B_los
+// is returned for the S0 slot (:449-450) and nothing is guarded.  Not
+// inlined: it runs once per pixel.
+__device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const float *spec,
+                                         bool init)
 {
     const int n = c->L;
     const double l0 = c->lambda0;
@@ -808,8 +816,10 @@ __device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const f
     double tan_gamma = fabs(sqrt(x / y));
     double gamma_rad = atan(tan_gamma);
     double gamma = gamma_rad * (180 / PM_PI);
-    if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (PM_PI / 180);
-    if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (PM_PI / 180);
+    if (init) {
+        if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (PM_PI / 180);
+        if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (PM_PI / 180);
+    }
     if (Blos < 0) gamma = (180) - gamma;
     double su = 0.0, sq = 0.0;
     for (int i = 0; i < n; i++) {
@@ -834,10 +844,13 @@ __device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const f
     r.vlos = Vlos;
     r.gm = gamma;
     r.az = phi;
-    // NaN guards of the pixel loop, milos.c:976-983
-    if (isnan(r.B)) r.B = 1;
-    if (isnan(r.vlos)) r.vlos = 1e-3;
-    if (isnan(r.gm)) r.gm = 1;
-    if (isnan(r.az)) r.az = 1;
+    r.blos = Blos;
+    if (init) {
+        // NaN guards of the pixel loop, milos.c:976-983
+        if (isnan(r.B)) r.B = 1;
+        if (isnan(r.vlos)) r.vlos = 1e-3;
+        if (isnan(r.gm)) r.gm = 1;
+        if (isnan(r.az)) r.az = 1;
+    }
     return r;
 }

@@ -520,8 +520,12 @@ pm_invert_kernel(InvArgs a)
             }
             wave_sync();
             nfree = 4 * L - c->NT - group_sum_int<W>(ng);     // :507-508
-            {
-                Classic e = classical(c, sobs);
+            if (c->no_classical) {
+                if (t == 0)
+                    for (int i = 0; i < 11; i++) lm[i] = lm[11 + i] = c->guess[i];
+                wave_sync();
+            } else {
+                Classic e = classical(c, sobs, true);
                 wave_sync();
                 if (t == 0) {
                     for (int i = 0; i < 11; i++) lm[i] = c->guess[i];
@@ -805,6 +809,25 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
     }
 }
 
+// ------------------------------------------------------------ estimates only
+
+// The classical-estimates mode (NumberOfCycles < 0, milos.c:474-500): the
+// guess with B, vlos, gamma, phi from estimacionesClasicas(forInitialUse = 0)
+// and B_los in the S0 slot; one thread per pixel.
+__global__ void pm_classical_kernel(const PmConst *c, const float *stokes, int n, double *models)
+{
+    const int pix = blockIdx.x * blockDim.x + threadIdx.x;
+    if (pix >= n) return;
+    const Classic e = classical(c, stokes + (size_t)pix * 4 * c->L, false);
+    double *m = models + (size_t)pix * 11;
+    for (int i = 0; i < 11; i++) m[i] = c->guess[i];
+    m[1] = e.B;
+    m[2] = e.vlos;
+    m[5] = e.gm;
+    m[6] = e.az;
+    m[7] = e.blos;
+}
+
 // ============================================================ host side
 
 struct pm_ctx {
@@ -952,6 +975,7 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     if (rc != PM_OK) { delete ctx; return rc; }
     c.L = L;
     c.NT = pm_nterms(cfg->fix, cfg->guess[9]);
+    c.no_classical = cfg->no_classical ? 1 : 0;
     if (cfg->slight) {
         c.use_slight = 1;
         for (int i = 0; i < 4 * L; i++) c.slight[i] = cfg->slight[i];
@@ -1193,6 +1217,20 @@ extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, do
 {
     return pm_invert_sl_device(ctx, d_stokes, nullptr, npix, d_models, d_chi2, d_iters,
                                d_accepted, d_fit, stream);
+}
+
+extern "C" int pm_classical_device(pm_ctx *ctx, const float *d_stokes, int npix,
+                                   double *d_models, void *stream)
+{
+    if (!ctx || npix < 0) return PM_EINVAL;
+    if (npix == 0) return PM_OK;
+    if (!d_stokes || !d_models) return PM_EINVAL;
+    hipStream_t st = (hipStream_t)stream;
+    ctx->last_stream = st;
+    hipLaunchKernelGGL(pm_classical_kernel, dim3((unsigned)((npix + 127) / 128)), dim3(128), 0, st,
+                       (const PmConst *)ctx->d, d_stokes, npix, d_models);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
 }
 
 extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
@@ -1446,4 +1484,33 @@ extern "C" int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *mod
                          int *iters, int *accepted, float *fit)
 {
     return pm_invert_sl(ctx, stokes, nullptr, npix, models, chi2, iters, accepted, fit);
+}
+
+extern "C" int pm_classical(pm_ctx *ctx, const float *stokes, int npix, double *models)
+{
+    if (!ctx || npix < 0) return PM_EINVAL;
+    if (npix == 0) return PM_OK;
+    if (!stokes || !models) return PM_EINVAL;
+    const size_t L = ctx->h.L, n = npix;
+    DevBuf ds, dm;
+    if (ds.alloc(sizeof(float) * 4 * L * n) || dm.alloc(sizeof(double) * 11 * n)) return PM_ENOMEM;
+    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
+    int rc = pm_classical_device(ctx, (const float *)ds.p, npix, (double *)dm.p, nullptr);
+    if (rc) return rc;
+    PM_HIP(hipDeviceSynchronize());
+    PM_HIP(hipMemcpy(models, dm.p, sizeof(double) * 11 * n, hipMemcpyDeviceToHost));
+    return PM_OK;
+}
+
+extern "C" int pm_ctx_autotune_host(pm_ctx *ctx, const float *stokes, const float *slight,
+                                    int npix, int chosen[3])
+{
+    if (!ctx || !stokes || npix < 1) return PM_EINVAL;
+    const size_t n = (size_t)ctx->h.L * 4 * npix;
+    DevBuf ds, dl;
+    if (ds.alloc(sizeof(float) * n) || (slight && dl.alloc(sizeof(float) * n))) return PM_ENOMEM;
+    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * n, hipMemcpyHostToDevice));
+    if (slight) PM_HIP(hipMemcpy(dl.p, slight, sizeof(float) * n, hipMemcpyHostToDevice));
+    return pm_ctx_autotune(ctx, (const float *)ds.p, slight ? (const float *)dl.p : nullptr, npix,
+                           nullptr, chosen);
 }

@@ -36,7 +36,7 @@ class Config(C.Structure):
                 ("noise", C.c_double), ("ilambda", C.c_double), ("toplim", C.c_double),
                 ("miter", C.c_int), ("mu", C.c_double), ("logclambda", C.c_int),
                 ("guess", C.c_double * NPAR), ("precision", C.c_int),
-                ("slight", C.POINTER(C.c_float))]
+                ("slight", C.POINTER(C.c_float)), ("no_classical", C.c_int)]
 
 
 class BatchItem(C.Structure):
@@ -80,6 +80,8 @@ def lib():
         L.pm_graph_launch.argtypes = [vp, vp]
         L.pm_graph_destroy.argtypes = [vp]
         L.pm_graph_destroy.restype = None
+        L.pm_classical.argtypes = [vp, vp, C.c_int, vp]
+        L.pm_classical_device.argtypes = [vp, vp, C.c_int, vp, vp]
         L.pm_server_create.argtypes = [C.POINTER(ServerOpts), C.POINTER(vp)]
         L.pm_server_destroy.argtypes = [vp]
         L.pm_server_destroy.restype = None
@@ -120,7 +122,8 @@ class Context:
 
     def __init__(self, lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
                  noise=1e-6, ilambda=float(np.float32(0.1)), toplim=1e-12, miter=50,
-                 mu=1.0, logclambda=0, guess=None, precision=PM_FP64, slight=None):
+                 mu=1.0, logclambda=0, guess=None, precision=PM_FP64, slight=None,
+                 classical=True):
         self._lam = np.ascontiguousarray(lam, dtype=np.float64)
         self._slight = None if slight is None else \
             np.ascontiguousarray(slight, dtype=np.float32).reshape(4, len(self._lam))
@@ -145,6 +148,7 @@ class Context:
         cfg.precision = precision
         if self._slight is not None:
             cfg.slight = self._slight.ctypes.data_as(C.POINTER(C.c_float))
+        cfg.no_classical = 0 if classical else 1
         h = C.c_void_p()
         _check(lib().pm_ctx_create(C.byref(cfg), C.byref(h)))
         self.h = h
@@ -201,6 +205,13 @@ class Context:
     def synth_ptrs(self, d_models, n, d_spectra, d_rf=0, stream=0):
         _check(lib().pm_synth_device(self.h, d_models, n, d_spectra, d_rf or None,
                                      stream or None))
+
+    def classical(self, stokes):
+        """Classical estimates only: models [n][11] (B_los in the S0 slot)."""
+        st = np.ascontiguousarray(stokes, dtype=np.float32)
+        models = np.zeros((st.shape[0], NPAR))
+        _check(lib().pm_classical(self.h, st.ctypes.data, st.shape[0], models.ctypes.data))
+        return models
 
     def autotune_ptrs(self, d_stokes, npix, stream=0, d_slight=0):
         """Times the kernel variants on npix pixels at device address

@@ -18,6 +18,35 @@ HIPCC_FLAGS = [
 ]
 
 
+CLI_DIR = os.path.join(os.path.dirname(HERE), "cli")
+IO_LIB = os.path.join(HERE, "libpmilos_io.so")
+CLI_BIN = os.path.join(HERE, "pmilos_gpu")
+CXX_FLAGS = ["-O2", "-std=c++17", "-Wall", "-Wno-unused-result"]
+
+
+def build_cli(verbose=False):
+    """The command line tool (host C++ on the C ABI) and its file-format
+    library, which the CPU tests load on their own."""
+    cxx = os.environ.get("CXX", "g++")
+    io_src = os.path.join(CLI_DIR, "pm_io.cpp")
+    cmds = [
+        [cxx] + CXX_FLAGS + ["-fPIC", "-shared", "-o", IO_LIB + ".tmp", io_src],
+        [cxx] + CXX_FLAGS + ["-o", CLI_BIN + ".tmp", os.path.join(CLI_DIR, "pmilos_cli.cpp"), io_src,
+                             "-L" + HERE, "-lpmilos_hip", "-Wl,-rpath,$ORIGIN", "-lpthread"],
+    ]
+    for cmd, out in zip(cmds, (IO_LIB, CLI_BIN)):
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        res = subprocess.run(cmd, capture_output=True, text=True)
+        if res.returncode != 0:
+            sys.stderr.write(res.stderr)
+            raise RuntimeError("%s failed (%d)" % (cxx, res.returncode))
+        if verbose and res.stderr.strip():
+            print(res.stderr)
+        os.replace(out + ".tmp", out)
+    return CLI_BIN
+
+
 def build(verbose=False):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = LIB + ".tmp"
@@ -33,6 +62,7 @@ def build(verbose=False):
     if verbose and errs:
         print("\n".join(errs))
     os.replace(tmp, LIB)
+    build_cli(verbose)
     return LIB
 
 

@@ -406,6 +406,38 @@ def test_classical_estimates_closed_form():
         np.testing.assert_allclose(m[6], phi, atol=1e-3)
 
 
+def test_classical_estimates_only_mode():
+    """forInitialUse = 0 (the estimates-only mode): B_los goes to the S0 slot
+    with its sign, and near 90 degrees of inclination B is |B_los| / cos(gamma)
+    of the estimated gamma itself, not of the 85/95-degree nudge the
+    inversion's starting point uses (milosUtils.c:395-402, 449-450)."""
+    s, lam, _ = setup_l30(psf=False)
+    l0 = FE6173
+    C = 4.6686411e-13 * l0 * l0 * 2.5
+    for blos in (800.0, -1500.0):
+        split = C * blos
+        Ipv = 1 - 0.4 * _gauss(lam, l0 + split, 0.035)
+        Imv = 1 - 0.4 * _gauss(lam, l0 - split, 0.035)
+        spec = np.stack([(Ipv + Imv) / 2, 0 * lam, 0 * lam, (Ipv - Imv) / 2]).astype(np.float32)
+        m0 = O.classical(s, spec, GUESS, for_init=False)
+        m1 = O.classical(s, spec, GUESS, for_init=True)
+        np.testing.assert_allclose(m0[7], blos, rtol=2e-4)
+        assert m1[7] == GUESS[7]
+        assert np.array_equal(m0[[1, 2, 5, 6]], m1[[1, 2, 5, 6]], equal_nan=True)
+    # strong linear polarisation, weak V: gamma between 85 and 90 degrees
+    A = 0.2 * _gauss(lam, l0, 0.05)
+    I = 1 - 0.4 * _gauss(lam, l0, 0.035)
+    split = C * 50.0
+    V = (0.4 * (_gauss(lam, l0 - split, 0.035) - _gauss(lam, l0 + split, 0.035))) / 2
+    spec = np.stack([I, A, 0 * A, V]).astype(np.float32)
+    m0 = O.classical(s, spec, GUESS, for_init=False)
+    m1 = O.classical(s, spec, GUESS, for_init=True)
+    g = m0[5]
+    assert 85 <= g <= 90, g
+    np.testing.assert_allclose(m0[1], min(4000.0, abs(m0[7]) / np.cos(np.radians(g))), rtol=1e-9)
+    np.testing.assert_allclose(m1[1], min(4000.0, abs(m0[7]) / np.cos(np.radians(85.0))), rtol=1e-9)
+
+
 def test_log10f_within_one_ulp_of_correct_rounding():
     """lm_mils' logclambda factor uses glibc log10f (milosUtils.c:621), which
     the oracle calls too.  The GPU library computes the correctly rounded
