@@ -204,7 +204,8 @@ def test_cli_watch_directory(work):
     for i in (1, 7):
         models = draw_models(rows * cols, 100 + i)
         cubes[i] = add_noise(ctx.synth(models), models, 1e-3, 110 + i)
-    fitsutil.write(str(d / "obs001.fits"), fitsutil.stokes_cube(cubes[1], rows, cols))
+    cty = ("WAVE-GRI", "STOKES", "HPLN-TAN", "HPLT-TAN")
+    fitsutil.write(str(d / "obs001.fits"), fitsutil.stokes_cube(cubes[1], rows, cols), ctypes=cty)
     mtrol(str(work / "w.mtrol"), 50, "data/obs", "l30.grid", psf=60)
     _minit(str(work / "w.minit"), "w.mtrol", "", t1="*", t2="*", save=0)
     say("case cli watch")
@@ -212,10 +213,15 @@ def test_cli_watch_directory(work):
                          cwd=work, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
     try:
         t0 = time.time()
-        while not (work / "obs001_mod.fits").exists() and time.time() - t0 < 60:
+        while not (work / "obs001_mod.fits").exists() and time.time() - t0 < 60 \
+                and p.poll() is None:
             time.sleep(0.1)
-        assert (work / "obs001_mod.fits").exists()
-        fitsutil.write(str(d / "obs007.fits.part"), fitsutil.stokes_cube(cubes[7], rows, cols))
+        if not (work / "obs001_mod.fits").exists():
+            out, err = p.communicate(timeout=60)
+            say(out[-2000:], err[-2000:])
+            raise AssertionError("no output for the first cube")
+        fitsutil.write(str(d / "obs007.fits.part"), fitsutil.stokes_cube(cubes[7], rows, cols),
+                       ctypes=cty)
         os.rename(d / "obs007.fits.part", d / "obs007.fits")
         out, err = p.communicate(timeout=60)
     finally:
