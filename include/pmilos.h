@@ -81,6 +81,13 @@ typedef struct {
                                     as lm_mils on a .per profile (milos.c:741-775);
                                     0: classical estimates and NaN guards first,
                                     as the FITS pixel loop (milos.c:958-983) */
+    int use_fft;                 /* useFFT (readConfig.c:995-999): 1 = the PSF
+                                    enters every Stokes row and response row by
+                                    circular convolution, and is folded into the
+                                    macroturbulence kernel circularly (the DFT
+                                    products of mil_sinrf.c:301-386 and
+                                    me_der.c:331-470); 0 = the zero-padded
+                                    correlation with the continuum trick for I */
 } pm_config;
 
 const char *pm_strerror(int status);

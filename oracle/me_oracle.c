@@ -376,11 +376,17 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
         mac_applied = 1;
         double gmac[ORC_MAXL];
         orc_gauss_mac(MC, s->lambda, L, s->lambda0, 0, gmac);
-        if (s->use_psf) orc_conv_direct(gmac, L, s->G, L);
+        if (s->use_psf && s->use_fft)                               /* :302-314 */
+            orc_conv_circular(gmac, s->G, L, gmac);
+        else if (s->use_psf)
+            orc_conv_direct(gmac, L, s->G, L);
         for (int st = 0; st < 4; st++)
             orc_conv_circular(spectra + L * st, gmac, L, spectra + L * st);
     }
-    if (!mac_applied && s->use_psf) {                               /* :387-402 */
+    if (!mac_applied && s->use_psf && s->use_fft) {                 /* :362-386 */
+        for (int st = 0; st < 4; st++)
+            orc_conv_circular(spectra + L * st, s->G, L, spectra + L * st);
+    } else if (!mac_applied && s->use_psf) {                        /* :387-402 */
         double ic = (spectra[0] > spectra[L - 1]) ? spectra[0] : spectra[L - 1];
         orc_conv_direct_ic(spectra, L, s->G, L, ic);
         for (int st = 1; st < 4; st++) orc_conv_direct(spectra + L * st, L, s->G, L);
@@ -543,7 +549,10 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
             double a = (s->lambda[i] - centro) / ild;
             gder[i] = (gmac[i] / MC * ((a * a) - 1.0));
         }
-        if (s->use_psf) {
+        if (s->use_psf && s->use_fft) {                             /* :331-354 */
+            orc_conv_circular(gder, s->G, L, gder);
+            orc_conv_circular(gmac, s->G, L, gmac);
+        } else if (s->use_psf) {
             orc_conv_direct(gder, L, s->G, L);
             orc_conv_direct(gmac, L, s->G, L);
         }
@@ -555,7 +564,13 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
                     orc_conv_circular(D + L * p + L * NT * st, gmac, L,
                                       D + L * p + L * NT * st);
     }
-    if (!mac_applied && s->use_psf) {                               /* :471-495 */
+    if (!mac_applied && s->use_psf && s->use_fft) {                 /* :441-470 */
+        for (int st = 0; st < 4; st++)
+            for (int p = 0; p < NT; p++)
+                if (p != 7)
+                    orc_conv_circular(D + L * p + L * NT * st, s->G, L,
+                                      D + L * p + L * NT * st);
+    } else if (!mac_applied && s->use_psf) {                        /* :471-495 */
         for (int p = 0; p < NT; p++) {
             if (p == 7) continue;
             double *row = D + L * p;

@@ -60,6 +60,12 @@ typedef struct {
      * them so the response functions become the exact derivatives of its
      * own synthesis, which finite differences can then pin. */
     int exact_rho_deriv;
+    /* useFFT (readConfig.c:995-999): the PSF enters by circular convolution
+     * (the DFT product of mil_sinrf.c:301-386, me_der.c:331-470) instead of
+     * the zero-padded correlation with the continuum trick.  Evaluated here
+     * as the direct circular sum, which the product equals in exact
+     * arithmetic. */
+    int use_fft;
 } orc_setup_t;
 
 /* ---- setup (run once) ---- */

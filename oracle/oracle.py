@@ -37,7 +37,7 @@ class Setup(C.Structure):
                 ("miter", C.c_int), ("ah", C.c_double),
                 ("logclambda", C.c_int), ("use_slight", C.c_int),
                 ("slight", C.c_double * (4 * MAXL)),
-                ("exact_rho_deriv", C.c_int)]
+                ("exact_rho_deriv", C.c_int), ("use_fft", C.c_int)]
 
 
 _lib = None
@@ -139,7 +139,7 @@ def gauss_psf(fwhm_mA, lam):
 
 def make_setup(lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
                noise=1e-6, ilambda=float(np.float32(0.1)), toplim=1e-12,
-               miter=50, ah=1.0, logclambda=0, guess_mac=0.0, slight=None):
+               miter=50, ah=1.0, logclambda=0, guess_mac=0.0, slight=None, use_fft=False):
     """quantum = (sl, ll, jl, su, lu, ju) as readFileCuanticLines returns."""
     s = Setup()
     L = len(lam)
@@ -166,6 +166,7 @@ def make_setup(lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
     s.miter = miter
     s.ah = ah
     s.logclambda = logclambda
+    s.use_fft = int(bool(use_fft))
     if slight is not None:
         s.use_slight = 1
         sl = np.asarray(slight, dtype=np.float64).ravel()

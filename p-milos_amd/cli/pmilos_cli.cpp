@@ -55,7 +55,6 @@ void usage()
             "  FITS cubes in any axis order are read by their CTYPEs (the reference's\n"
             "  fast path scrambles HPLN,HPLT,WAVE,STOKES cubes); a blank output prefix\n"
             "  is no prefix (not \"!\"); PSF samples outside a .psf file's range are 0;\n"
-            "  the FFT convolution switch is read and the direct convolution is used;\n"
             "  the directory watch skips every name containing _mod or _stokes.\n");
 }
 
@@ -207,6 +206,7 @@ pm_ctx *make_ctx(const Setup &s, const Opts &o, const float *slight, bool classi
     cfg.precision = o.precision;
     cfg.slight = slight;
     cfg.no_classical = classical ? 0 : 1;
+    cfg.use_fft = s.c.use_fft ? 1 : 0;
     pm_ctx *ctx = nullptr;
     int rc = pm_ctx_create(&cfg, &ctx);
     if (rc) {
@@ -629,6 +629,5 @@ int main(int argc, char **argv)
     if (rc == -3) return fail("the wavelength or PSF file is of an unknown kind", file);
     if (rc == -4) return fail("a subfield needs all of subx1, subx2, suby1, suby2", file);
     if (setup(s)) return 1;
-    if (s.c.use_fft) fprintf(stderr, "pmilos_gpu: FFT convolution requested; using the direct convolution\n");
     return minit ? run_pmilos(s, o) : run_milos(s, o);
 }

@@ -49,6 +49,10 @@ struct PmConst {
     // start every pixel from the guess itself, without classical estimates
     // (lm_mils on a .per profile, milos.c:741-775)
     int no_classical;
+    // PSF by circular convolution (the reference's FFT mode): Stokes and
+    // response rows alike, I without the continuum trick, and folded into
+    // the macroturbulence kernel circularly
+    int use_fft;
 };
 
 // ------------------------------------------------------------ group helpers

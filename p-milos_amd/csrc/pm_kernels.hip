@@ -127,13 +127,14 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
     wave_sync();
 }
 
-// Circular convolution with the kernel h (LDS, length L), rotated so that
+// Circular convolution with the kernel h (length L: a kernel in LDS, or the
+// PSF of the constants in the FFT mode), rotated so that
 // out[m] = conv[(m + L/2) mod L] (convCircular, convolution.c:126-153), over
 // the kernel's taps [jlo, jhi): the others are below the rounding of the
 // sum (see Taps).  The rows are staged in the middle of the PSF row buffer,
 // leaving its zero margins untouched.
-template <typename T, int W, int NLPL, int R>
-__device__ __forceinline__ void convC(T *pad, const T *h, int jlo, int jhi, int t, int L,
+template <typename T, int W, int NLPL, int R, typename H = T>
+__device__ __forceinline__ void convC(T *pad, const H *h, int jlo, int jhi, int t, int L,
                                       T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
@@ -171,6 +172,24 @@ __device__ __forceinline__ void convC(T *pad, const T *h, int jlo, int jhi, int 
     wave_sync();
 }
 
+// The PSF applied to R rows: zero-padded correlation with the continuum
+// trick on I (direct mode), or circular convolution of every row (FFT
+// mode, mil_sinrf.c:362-386, me_der.c:441-470: the DFT product with the
+// PSF's transform and the -L/2 rotation is convCircular with the PSF as
+// kernel).  The FFT mode is handled by the general form only.
+template <typename T, int W, int NLPL, int R, bool GEN, bool IC = true>
+__device__ __forceinline__ void convPSF(const PmConst *__restrict__ c, T *pad, int t, int L,
+                                        T x[R][NLPL])
+{
+    if (GEN && c->use_fft) {
+        const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
+        const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
+        convC<T, W, NLPL, R, double>(pad, c->G, jlo, jhi, t, L, x);
+    } else {
+        convR<T, W, NLPL, R, IC>(c, pad, t, L, x);
+    }
+}
+
 // Tap windows of the two macroturbulence kernels: [lo, hi) holds every tap
 // whose magnitude reaches 1e-17 (FP64 form) / 1e-9 (FP32 form) of the
 // kernel's largest; the rest are below the rounding of the sums they enter.
@@ -200,8 +219,9 @@ __device__ __forceinline__ void tap_window(const T v[NLPL], int t, int L, int &l
 
 // Macroturbulence kernel of model m and its v_mac derivative (fgauss,
 // fgauss.c:16-66; the derivative as me_der.c:328-329,410), both correlated
-// with the PSF when there is one (mil_sinrf.c:348-350, me_der.c:412-416);
-// written to gm / gd in LDS, with their tap windows.
+// with the PSF when there is one (mil_sinrf.c:348-350, me_der.c:412-416),
+// or circularly convolved with it in the FFT mode (mil_sinrf.c:302-314,
+// me_der.c:331-354); written to gm / gd in LDS, with their tap windows.
 template <typename T, int W, int NLPL>
 __device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const double *m,
                                             T *pad, int t, int L, T *gm, T *gd, Taps &tp)
@@ -231,7 +251,7 @@ __device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const
         x[0][k] = T(g);
         x[1][k] = T(g / MC * (a2[k] - 1.0));
     }
-    if (c->use_psf) convR<T, W, NLPL, 2, false>(c, pad, t, L, x);
+    if (c->use_psf) convPSF<T, W, NLPL, 2, true, false>(c, pad, t, L, x);
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
@@ -295,7 +315,7 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
         mac_kernels<T, W, NLPL>(c, m, pad, t, L, gm, gd, ex.tp);
         convC<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, S);   // :353-354
     } else if (c->use_psf) {
-        convR<T, W, NLPL, 4>(c, pad, t, L, S);                     // :387-402
+        convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, S);              // :362-402
     }
     if (GEN && slp) {                                              // :405-412
         const T alf = T(m[10]);
@@ -363,7 +383,7 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
                     for (int s = 0; s < 4; s++) x[4 * u + s][k] = D[k][s][p];
             }
             if (mac) convC<T, W, NLPL, R>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
-            else convR<T, W, NLPL, R>(c, pad, t, L, x);
+            else convPSF<T, W, NLPL, R, GEN>(c, pad, t, L, x);
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
                 if (q0 + u >= NPS) continue;
@@ -976,6 +996,7 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     c.L = L;
     c.NT = pm_nterms(cfg->fix, cfg->guess[9]);
     c.no_classical = cfg->no_classical ? 1 : 0;
+    c.use_fft = cfg->use_fft ? 1 : 0;
     if (cfg->slight) {
         c.use_slight = 1;
         for (int i = 0; i < 4 * L; i++) c.slight[i] = cfg->slight[i];
@@ -1136,11 +1157,12 @@ static int variants(const pm_ctx *ctx, bool gen, InvVariant *v)
                                       : variants_of<double, 9, false>(L, v);
 }
 
-// The 9-row kernel covers NTERMS 9 without stray light; everything else
-// takes the general one.
+// The 9-row kernel covers NTERMS 9 without stray light, with the PSF by
+// zero-padded correlation; everything else takes the general one.
 static bool needs_general(const pm_ctx *ctx, bool slight_cube)
 {
-    return ctx->h.NT != 9 || ctx->h.use_slight || slight_cube;
+    return ctx->h.NT != 9 || ctx->h.use_slight || slight_cube ||
+           (ctx->h.use_fft && ctx->h.use_psf);
 }
 
 extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectra,

@@ -36,7 +36,8 @@ class Config(C.Structure):
                 ("noise", C.c_double), ("ilambda", C.c_double), ("toplim", C.c_double),
                 ("miter", C.c_int), ("mu", C.c_double), ("logclambda", C.c_int),
                 ("guess", C.c_double * NPAR), ("precision", C.c_int),
-                ("slight", C.POINTER(C.c_float)), ("no_classical", C.c_int)]
+                ("slight", C.POINTER(C.c_float)), ("no_classical", C.c_int),
+                ("use_fft", C.c_int)]
 
 
 class BatchItem(C.Structure):
@@ -123,7 +124,7 @@ class Context:
     def __init__(self, lambda0, quantum, lam, psf=None, fix=None, weight=(1, 1, 1, 1),
                  noise=1e-6, ilambda=float(np.float32(0.1)), toplim=1e-12, miter=50,
                  mu=1.0, logclambda=0, guess=None, precision=PM_FP64, slight=None,
-                 classical=True):
+                 classical=True, use_fft=False):
         self._lam = np.ascontiguousarray(lam, dtype=np.float64)
         self._slight = None if slight is None else \
             np.ascontiguousarray(slight, dtype=np.float32).reshape(4, len(self._lam))
@@ -149,6 +150,7 @@ class Context:
         if self._slight is not None:
             cfg.slight = self._slight.ctypes.data_as(C.POINTER(C.c_float))
         cfg.no_classical = 0 if classical else 1
+        cfg.use_fft = 1 if use_fft else 0
         h = C.c_void_p()
         _check(lib().pm_ctx_create(C.byref(cfg), C.byref(h)))
         self.h = h

@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 GRIDS = {30: (FE6173, -290, 20, 290), 49: (FE6173, -600, 25, 600),
-         112: (FE6302, -1193.25, 21.5, 1193.25)}
+         112: (FE6302, -1193.25, 21.5, 1193.25), 31: (FE6173, -290, 20, 310)}
 
 # (fix, guess v_mac, stray light) of each NTERMS layout the reference has
 # (milos.c:154-157; me_der.c:418-425 and :499-525 for where rows 9/10 go)
@@ -27,6 +27,7 @@ CASES = {
     "ff_nt10_sl": ([1] * 9 + [0, 1], 0.0, True),    # no v_mac, filling factor row 9
     "macfix_nt10_sl": ([1] * 9 + [0, 0], 0.5, True),  # v_mac fixed but applied; row 9
     "nt9_sl": ([1] * 9 + [0, 0], 0.0, True),        # alpha fixed, stray light mixed in
+    "nt9": ([1] * 9 + [0, 0], 0.0, False),          # the 9-parameter layout
 }
 
 
@@ -228,3 +229,52 @@ def test_invert_per_pixel_stray_light():
     a = ctx.invert(cube, slight=rep)
     b = ctx.invert(cube)
     assert np.array_equal(a["model"], b["model"]) and np.array_equal(a["iters"], b["iters"])
+
+
+# the FFT convolution mode (useFFT): even and odd grids, PSF alone and
+# folded into the macroturbulence kernel, with stray light
+FFT_CASES = [("nt9", 30), ("nt9", 31), ("mac_nt10", 30), ("mac_nt11_sl", 49)]
+
+
+@pytest.mark.parametrize("case,L", FFT_CASES)
+def test_synthesis_and_rf_fft_mode(case, L):
+    """useFFT: the PSF by circular convolution of every Stokes and response
+    row, and folded into the macroturbulence kernel circularly
+    (mil_sinrf.c:301-386, me_der.c:331-470), against the oracle's FFT mode:
+    FP64 within 1e-10, the mixed form within its stated tolerances.  The
+    PSF-only models differ from the direct mode's near the window edges."""
+    say("case fft synth %s L=%d" % (case, L))
+    s, ctx, guess, sl = _pair(case, L, True, use_fft=True)
+    _, ctxm, _, _ = _pair(case, L, True, precision=pmilos.PM_MIXED, use_fft=True)
+    _, ctxd, _, _ = _pair(case, L, True)
+    models = _models(40, 5000 + L, case)
+    spec_g, rf_g = ctx.synth(models, rf=True)
+    spec_m, rf_m = ctxm.synth(models, rf=True)
+    spec_d = ctxd.synth(models)
+    for i, m in enumerate(models):
+        sp, rf = O.synth(s, m, rf=True)
+        e1, e2 = _err_spectra(spec_g[i], sp), _err_rf(rf_g[i], rf)
+        assert e1 < 1e-10, (i, e1)
+        assert e2 < 1e-10, (i, e2)
+        em = np.abs(spec_m[i] - sp).max() / np.abs(sp[0]).max()
+        assert em < 1e-6, (i, em)
+        erm = _err_rf(rf_m[i], rf)
+        assert erm < 2e-3, (i, erm)
+        if m[9] <= 1e-4:
+            assert np.abs(spec_d[i] - spec_g[i]).max() > 1e-9, i
+
+
+@pytest.mark.parametrize("precision", [pmilos.PM_FP64, pmilos.PM_MIXED])
+def test_invert_fft_mode(precision):
+    """FFT-mode inversions of a seeded 30-sample cube (sigma 1e-3 Ic)
+    against the oracle's FFT mode."""
+    n = 256
+    say("case fft invert nt9 L=30 precision=%d npix=%d" % (precision, n))
+    s, ctx, guess, sl = _pair("nt9", 30, True, precision=precision, use_fft=True)
+    truths = _models(n, 6030, "nt9")
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=37)
+    o = O.invert(s, cube, guess)
+    g = ctx.invert(cube)
+    tol = TOL_GEN if precision == pmilos.PM_FP64 else \
+        np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 1e-2, 1e-2, 1e-2, 1e-2])
+    _compare(g, o, "fft nt9 L30 p%d" % precision, tol, 0.98 if precision == 0 else 0.97)

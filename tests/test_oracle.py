@@ -259,6 +259,137 @@ def test_response_functions_psf_is_row_convolution():
             np.testing.assert_allclose(rfp[st, p], row, rtol=1e-13, atol=1e-16)
 
 
+# FFTW's unnormalised transforms (FFTW_FORWARD / FFTW_BACKWARD) in numpy
+def _fftw_fwd(x):
+    return np.fft.fft(x)
+
+
+def _fftw_bwd(x):
+    return np.fft.ifft(x) * len(x)
+
+
+def _fft_rotate(y):
+    """The rotation after each backward transform: the first n/2 outputs go
+    to n/2 (+1 when n is odd) onwards, the rest to the start
+    (mil_sinrf.c:326-331)."""
+    n = len(y)
+    out = np.empty_like(y)
+    st = n // 2 + n % 2
+    out[st:st + n // 2] = y[:n // 2]
+    out[:n - n // 2] = y[n // 2:]
+    return out
+
+
+def _fft_psf_row(row, G):
+    """mil_sinrf.c:362-386 / me_der.c:441-470 with fftw_G_PSF = FFT(G)/n
+    (milos.c:373-380)."""
+    n = len(row)
+    y = _fftw_bwd((_fftw_fwd(row) / n) * (_fftw_fwd(G) / n))
+    return _fft_rotate(y.real * n)
+
+
+def _fft_mac_filter(g, G):
+    """The transformed macroturbulence filter, folded with the PSF through
+    the DFT when there is one (mil_sinrf.c:301-324, me_der.c:331-360)."""
+    n = len(g)
+    if G is None:
+        filt = g.astype(complex)
+    else:
+        filt = _fft_rotate(_fftw_bwd((_fftw_fwd(G) / n) * (_fftw_fwd(g) / n)) * n)
+    return _fftw_fwd(filt)
+
+
+def _fft_mac_row(row, F):
+    n = len(row)
+    y = _fftw_bwd((_fftw_fwd(row) / n) * (F / n))
+    return _fft_rotate(y.real * n)
+
+
+@pytest.mark.parametrize("L", [30, 31])
+def test_fft_mode_is_the_dft_product(L):
+    """useFFT: the oracle's circular sums against the reference's DFT steps
+    restated with numpy's FFT, spectra and response functions, with a skewed
+    PSF (so convolution and correlation differ) on an even and an odd grid
+    (the rotation differs), PSF alone and PSF with macroturbulence."""
+    lam = O.grid(FE6173, -290, 20, -290 + 20 * (L - 1))
+    assert len(lam) == L
+    G = O.gauss_psf(60.0, lam) * (1 + 0.6 * np.linspace(-1, 1, L))
+    G /= G.sum()
+    for fix, gmac, m9 in (([1] * 9 + [0, 0], 0.0, 0.0), ([1] * 10 + [0], 0.5, 1.1),
+                          ([1] * 10 + [0], 0.5, 0.3)):
+        s = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=fix, guess_mac=gmac, use_fft=True)
+        s_raw = O.make_setup(FE6173, TRIPLET, lam, fix=fix, guess_mac=gmac)
+        NT = s.nterms
+        for m in (np.array([8., 1200., 0.7, 0.03, 0.2, 60., 30., 0.25, 0.72, m9, 1.]),
+                  np.array([25., 300., -1.1, 0.045, 0.08, 150., 160., 0.1, 0.9, m9, 1.])):
+            sp, rf = O.synth(s, m, rf=True)
+            m0 = m.copy()
+            m0[9] = 0
+            raw, raw_rf = O.synth(s_raw, m0, rf=True)
+            want_rf = raw_rf.copy()
+            if m9 == 0:
+                want = np.array([_fft_psf_row(raw[st], G) for st in range(4)])
+                for st in range(4):
+                    for p in range(NT):
+                        if p != 7:
+                            want_rf[st, p] = _fft_psf_row(raw_rf[st, p], G)
+            else:
+                g = np.zeros(L)
+                O.lib().orc_gauss_mac(m9, lam, L, FE6173, 0, g)
+                a = (lam - lam[L // 2]) / (FE6173 * m9 / 2.99792458e5)
+                F = _fft_mac_filter(g, G)
+                Fd = _fft_mac_filter(g / m9 * (a * a - 1.0), G)
+                want = np.array([_fft_mac_row(raw[st], F) for st in range(4)])
+                for st in range(4):
+                    want_rf[st, 9] = _fft_mac_row(raw[st], Fd)
+                    for p in range(9):
+                        if p != 7:
+                            want_rf[st, p] = _fft_mac_row(raw_rf[st, p], F)
+            for st in range(4):
+                sc = np.abs(want[st]).max()
+                assert np.abs(sp[st] - want[st]).max() <= 1e-12 * sc, (m9, st)
+                for p in range(NT):
+                    sc = max(np.abs(want_rf[st, p]).max(), 1e-300)
+                    assert np.abs(rf[st, p] - want_rf[st, p]).max() <= 1e-12 * sc, (m9, st, p)
+
+
+def test_fft_mode_against_direct_mode():
+    """With a symmetric PSF the circular and the zero-padded
+    (continuum-trick) PSF differ only near the window's edges, where the
+    circular one wraps the other edge's line wing in: below 1e-3 of the
+    continuum there and 1e-5 inside, yet not equal.  With macroturbulence
+    they differ only through the kernel folding (the PSF times the wrapped
+    kernel tails, below rounding here).  A rotation off by one sample moves
+    the line by a grid step: differences of order 0.1."""
+    lam = O.grid(FE6173, -290, 20, 290)
+    G = O.gauss_psf(60.0, lam)
+    for fix, gmac, m9 in (([1] * 9 + [0, 0], 0.0, 0.0), ([1] * 10 + [0], 0.5, 0.6)):
+        sd = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=fix, guess_mac=gmac)
+        sf = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=fix, guess_mac=gmac, use_fft=True)
+        m = np.array([8., 1200., 0.7, 0.03, 0.2, 60., 30., 0.25, 0.72, m9, 1.])
+        a, b = O.synth(sd, m), O.synth(sf, m)
+        d = np.abs(a - b)
+        if m9 == 0:
+            assert 0 < d.max() < 1e-3, d.max()
+            assert d[:, 5:-5].max() < 1e-5, d[:, 5:-5].max()
+        else:
+            assert d.max() < 1e-12, d.max()
+        for sh in (1, -1):
+            assert np.abs(np.roll(b, sh, axis=1) - a).max() > 2e-2
+
+
+def test_lm_fft_mode_recovers_truth():
+    """An FFT-mode inversion of FFT-mode noiseless data from a perturbed
+    guess recovers the model (the normal equations use the FFT-mode
+    response functions)."""
+    lam = O.grid(FE6173, -290, 20, 290)
+    s = O.make_setup(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam), use_fft=True)
+    truth = np.array([9., 900., 0.8, 0.035, 0.15, 70., 40., 0.25, 0.72, 0, 1.])
+    spec = O.synth(s, truth).astype(np.float32)
+    r = O.lm(s, spec, truth * np.array([1.2, 0.9, 1.1, 1.1, 0.9, 1.05, 0.95, 1.1, 0.95, 1, 1]))
+    np.testing.assert_allclose(r["model"][:9], truth[:9], rtol=2e-3, atol=1e-3)
+
+
 def test_pinv_against_numpy():
     rng = np.random.default_rng(11)
     for nt in (9, 10, 11):
