@@ -18,6 +18,7 @@
 
 #define PM_MAXC 16          // Zeeman components per group (pi / blue / red)
 #define PM_LMAX 128
+#define PM_HC 12            // PSF half-width of the register-blocked correlation
 #define PM_PI 3.14159265358979323846264338327950288419716939937510
 #define PM_RR 0.5641895836          // defines.h:73 (truncated 1/sqrt(pi))
 #define PM_VLIGHT 2.99792458e+5     // defines.h:75
@@ -42,6 +43,12 @@ struct PmConst {
     // PSF taps [glo, ghi) whose weight reaches 1e-17 (FP64 form) / 1e-9
     // (FP32 form) of the peak: the others are below the rounding of the sum
     int glo64, ghi64, glo32, ghi32;
+    // the same taps centred on the kernel centre: gcNN[PM_HC + d] = G[L/2 + d]
+    // inside the form's tap range, 0 outside; reachNN = the largest |d| with
+    // a tap (PM_HC + 1 when some tap lies beyond PM_HC)
+    double gc64[2 * PM_HC + 1];
+    float gc32[2 * PM_HC + 1];
+    int reach64, reach32;
     // stray-light profile [4][L] mixed in with the filling factor
     // (mil_sinrf.c:405-412), when use_slight
     int use_slight;

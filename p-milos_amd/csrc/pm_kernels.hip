@@ -62,6 +62,15 @@ struct Lds {
 
 // ------------------------------------------------------------ convolution
 
+// A uniform read of the constants through the constant address space, so
+// that it is a scalar load (the constants are not written while a kernel
+// runs).
+template <typename X>
+__device__ __forceinline__ X cld(const X *p)
+{
+    return *(const __attribute__((address_space(4))) X *)p;
+}
+
 // Zero-padded "same" correlation of R rows x[r][slot] with the PSF, in place
 // (convolution.c:54-78).  Rows 0 and (when R == 8) 4 are Stokes I rows and
 // use the continuum trick of convolution.c:90-113 with Ic = max(first, last)
@@ -112,7 +121,7 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
             for (int r = 0; r < R; r++) acc[r] = 0;
             const T *p0 = pad + lam;
             for (int j = jlo; j < jhi; j++) {
-                A gj = A(c->G[j]);
+                A gj = A(cld(c->G + j));
 #pragma unroll
                 for (int r = 0; r < R; r++) acc[r] += gj * (A)p0[r * PADW + j];
             }
@@ -123,6 +132,115 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
                 else x[r][k] = T(acc[r]);
             }
         }
+    }
+    wave_sync();
+}
+
+// Half-width of the register-blocked correlation for rows of NX samples
+// (the window start half - HC stays inside the row buffer for L/2 >= HC):
+// 12 taps each side in the FP64 form, 9 in the FP32 form, whose tap range
+// ends at 1e-9 of the peak (a 60 mA Gaussian on a 20 mA grid needs 11 and 8)
+template <typename T, int NX> struct ConvHC {
+    static constexpr int cap = sizeof(T) == 8 ? PM_HC : 9;
+    static constexpr int value = NX / 2 - 1 < cap ? NX / 2 - 1 : cap;
+};
+
+// The correlation of convR, register-blocked: each lane of the group takes
+// one block of B = R*NX/W consecutive outputs of one row, loads the B + 2*HC
+// samples it needs from the row buffer once, and accumulates the taps from
+// registers (a tap-outer loop over compile-time offsets, with the taps in
+// scalar registers).  Same operations in the same order as convR (taps in
+// ascending order, each product added to one accumulator), so the results
+// are the same bits.  The outputs go back to the row buffer in place: every
+// lane has loaded its window before any lane stores.
+template <typename T, int W, int NLPL, int R>
+struct ConvF {
+    typedef typename AccT<T>::type A;
+    static constexpr int NX = W * NLPL;
+    static constexpr int HC = ConvHC<T, NX>::value;
+    static constexpr int NB = W / R;                 // blocks per row
+    static constexpr int B = NX / NB;                // outputs per lane
+    static constexpr int NWIN = B + 2 * HC;
+    // used where the window and the accumulators stay well inside the
+    // register file
+    static constexpr bool ok = W % R == 0 && HC > 0 &&
+        (NWIN * (int)sizeof(T) + B * (int)sizeof(A)) / 4 <= 96;
+};
+
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ bool convF_usable(const PmConst *__restrict__ c, int L)
+{
+    using CF = ConvF<T, W, NLPL, R>;
+    const int reach = sizeof(T) == 8 ? c->reach64 : c->reach32;
+    return CF::ok && reach <= CF::HC && CF::HC <= L / 2;
+}
+
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int t, int L,
+                                      T x[R][NLPL])
+{
+    using CF = ConvF<T, W, NLPL, R>;
+    typedef typename CF::A A;
+    constexpr int PADW = 2 * W * NLPL, HC = CF::HC, NB = CF::NB, B = CF::B;
+    const int half = L / 2;
+    const int klast = (L - 1) / W, tlast = (L - 1) % W;
+    T ic[2] = {T(0), T(0)};
+#pragma unroll
+    for (int q = 0; q < (R == 8 ? 2 : 1); q++) {
+        const int r = 4 * q;
+        T last = x[r][0];
+#pragma unroll
+        for (int k = 1; k < NLPL; k++) if (k == klast) last = x[r][k];
+        T a = __shfl(x[r][0], 0, W), b = __shfl(last, tlast, W);
+        ic[q] = (a > b) ? a : b;
+    }
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L) {
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                T v = x[r][k];
+                if (r == 0) v = ic[0] - v;
+                if (R == 8 && r == 4) v = ic[1] - v;
+                pad[r * PADW + half + lam] = v;
+            }
+        }
+    }
+    wave_sync();
+    {
+        const int r = t / NB, b = t % NB;
+        const T *src = pad + r * PADW + half - HC + b * B;
+        T w[CF::NWIN];
+#pragma unroll
+        for (int i = 0; i < CF::NWIN; i++) w[i] = src[i];
+        A acc[B];
+#pragma unroll
+        for (int o = 0; o < B; o++) acc[o] = 0;
+        const T *gt = sizeof(T) == 8 ? (const T *)c->gc64 : (const T *)c->gc32;
+        // every tap of the window; those beyond the PSF's tap range are 0
+        // and add exactly 0
+        A g[2 * HC + 1];
+#pragma unroll
+        for (int d = -HC; d <= HC; d++) g[HC + d] = (A)cld(gt + PM_HC + d);
+#pragma unroll
+        for (int d = -HC; d <= HC; d++)
+#pragma unroll
+            for (int o = 0; o < B; o++) acc[o] += g[HC + d] * (A)w[o + HC + d];
+        const bool irow = (r % 4) == 0;
+        const T icr = (R == 8 && r == 4) ? ic[1] : ic[0];
+        T *dst = pad + r * PADW + half + b * B;
+#pragma unroll
+        for (int o = 0; o < B; o++)
+            if (b * B + o < L) dst[o] = irow ? T((A)icr - acc[o]) : T(acc[o]);
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam < L)
+#pragma unroll
+            for (int r = 0; r < R; r++) x[r][k] = pad[r * PADW + half + lam];
     }
     wave_sync();
 }
@@ -185,6 +303,8 @@ __device__ __forceinline__ void convPSF(const PmConst *__restrict__ c, T *pad, i
         const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
         const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
         convC<T, W, NLPL, R, double>(pad, c->G, jlo, jhi, t, L, x);
+    } else if (!GEN && IC && ConvF<T, W, NLPL, R>::ok && convF_usable<T, W, NLPL, R>(c, L)) {
+        convF<T, W, NLPL, R>(c, pad, t, L, x);
     } else {
         convR<T, W, NLPL, R, IC>(c, pad, t, L, x);
     }
@@ -353,7 +473,9 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
         unno<T>(g, sp[k], r);
         response<T, NT>(c, g, sp[k], r, D[k]);
     }
-    constexpr int R = Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
+    // rows per PSF pass: the buffer's 8 where they fit, 4 in the 9-row FP32
+    // form (its register-blocked windows stay small)
+    constexpr int R = (!GEN && sizeof(T) == 4) ? 4 : Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
     constexpr int PPC = R / 4;
     constexpr int NPS = 8;                      // rows 0..8 but S0's
     const bool mac = GEN && ex.mac;
@@ -471,7 +593,7 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // does, so alpha, beta and delta agree with the reference's NTERMS-sized
 // ones on the rows it has.  nfree uses the reference's NTERMS.
 template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
-__global__ void __launch_bounds__(BLK, (NLPL == 1 && !GEN ? 2 : 1))
+__global__ void __launch_bounds__(BLK, ((NLPL == 1 || sizeof(T) == 4) && !GEN ? 2 : 1))
 pm_invert_kernel(InvArgs a)
 {
     using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
@@ -1029,6 +1151,17 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
         }
         if (!c.use_psf) { lo64 = lo32 = 0; hi64 = hi32 = 0; }
         c.glo64 = lo64; c.ghi64 = hi64; c.glo32 = lo32; c.ghi32 = hi32;
+        const int half = L / 2;
+        c.reach64 = c.reach32 = 0;
+        for (int d = -PM_HC; d <= PM_HC; d++) {
+            const int j = half + d;
+            c.gc64[PM_HC + d] = (j >= lo64 && j < hi64) ? c.G[j] : 0.0;
+            c.gc32[PM_HC + d] = (j >= lo32 && j < hi32) ? (float)c.G[j] : 0.0f;
+        }
+        for (int j = lo64; j < hi64; j++) c.reach64 = std::max(c.reach64, std::abs(j - half));
+        for (int j = lo32; j < hi32; j++) c.reach32 = std::max(c.reach32, std::abs(j - half));
+        c.reach64 = std::min(c.reach64, PM_HC + 1);
+        c.reach32 = std::min(c.reach32, PM_HC + 1);
     }
     ctx->precision = cfg->precision;
     if (hipGetDevice(&ctx->device) != hipSuccess) { delete ctx; return PM_EDEVICE; }
