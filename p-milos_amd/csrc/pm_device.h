@@ -606,14 +606,18 @@ __device__ __forceinline__ void check_model(double *m)
 // smallest eigenvalue) exceeds 1e-9, every eigenvalue is far above the cutoff
 // and the pseudo-inverse is the inverse.  Otherwise returns false and the
 // caller uses the eigen pseudo-inverse itself.
+// An inactive row (zero diagonal) gets a unit pivot with its row and column
+// cleared, so its component comes out 0; that is done up front (rarely
+// needed), and the factorisation itself forms each row's products L_jk d_k
+// once for its pivot and the rows below.
 template <int N>
 __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower alpha, then beta */,
                                           double damp, double *x)
 {
     constexpr int P = N * (N + 1) / 2;
-    // F holds L (strictly lower, unit diagonal implied) with D on the diagonal
     double F[P], id[N];
     bool act[N];
+    bool all = true;
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int ii = i * (i + 1) / 2;
@@ -621,22 +625,27 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         for (int j = 0; j < i; j++) F[ii + j] = ab[ii + j];
         F[ii + i] = ab[ii + i] * damp;                       // milosUtils.c:559-563
         act[i] = F[ii + i] != 0.0;
+        all = all && act[i];
+    }
+    if (!all) {
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            const int ii = i * (i + 1) / 2;
+#pragma unroll
+            for (int j = 0; j < i; j++) F[ii + j] = (act[i] && act[j]) ? F[ii + j] : 0.0;
+            F[ii + i] = act[i] ? F[ii + i] : 1.0;
+        }
     }
 #pragma unroll
     for (int j = 0; j < N; j++) {
         const int jj = j * (j + 1) / 2;
-        if (!act[j]) {
-            F[jj + j] = 1.0;
-            id[j] = 1.0;
-#pragma unroll
-            for (int k = 0; k < j; k++) F[jj + k] = 0.0;
-#pragma unroll
-            for (int i = j + 1; i < N; i++) F[i * (i + 1) / 2 + j] = 0.0;
-            continue;
-        }
+        double w[N > 1 ? N - 1 : 1];
         double dj = F[jj + j];
 #pragma unroll
-        for (int k = 0; k < j; k++) dj -= F[jj + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
+        for (int k = 0; k < j; k++) {
+            w[k] = F[jj + k] * F[k * (k + 1) / 2 + k];
+            dj -= F[jj + k] * w[k];
+        }
         if (!(dj > 0.0)) return false;
         F[jj + j] = dj;
         const double inv_dj = 1.0 / dj;
@@ -646,11 +655,10 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
             const int ii = i * (i + 1) / 2;
             double v = F[ii + j];
 #pragma unroll
-            for (int k = 0; k < j; k++) v -= F[ii + k] * F[jj + k] * F[k * (k + 1) / 2 + k];
-            F[ii + j] = act[i] ? v * inv_dj : 0.0;
+            for (int k = 0; k < j; k++) v -= F[ii + k] * w[k];
+            F[ii + j] = v * inv_dj;
         }
     }
-    // solve L D L^T x = beta
 #pragma unroll
     for (int i = 0; i < N; i++) {
         const int ii = i * (i + 1) / 2;
@@ -669,8 +677,8 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         x[i] = act[i] ? v : 0.0;
     }
     // L^-1 in place (column by column), then trace(A^-1) =
-    // sum_k (1/d_k) sum_{i<=k} (L^-1)_{ki}^2; 1/trace bounds the smallest
-    // eigenvalue from below
+    // sum_k (1/d_k) sum_{i<=k} (L^-1)_{ki}^2 over the active rows; 1/trace
+    // bounds the smallest eigenvalue from below
 #pragma unroll
     for (int j = 0; j < N; j++) {
 #pragma unroll
@@ -685,12 +693,11 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
     double tr = 0.0;
 #pragma unroll
     for (int k = 0; k < N; k++) {
-        if (!act[k]) continue;
         const int kk = k * (k + 1) / 2;
         double s = 1.0;
 #pragma unroll
         for (int i = 0; i < k; i++) s += F[kk + i] * F[kk + i];
-        tr += s * id[k];
+        tr += act[k] ? s * id[k] : 0.0;
     }
     return tr < 1e9;
 }
