@@ -36,12 +36,15 @@ template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM
 template <typename T, int W, int NLPL, int NT, bool GEN = false, int BLK = BlockOf<W>::value>
 struct Lds {
     static constexpr int PADW = 2 * W * NLPL;                 // >= 2L
+    // row stride: odd, so that lanes reading the same offset of different
+    // rows fall in different LDS banks
+    static constexpr int RS = PADW + 1;
     static constexpr int CH = W >= 16 ? 16 : W;
     static constexpr int RED_BYTES = (CH + 1) * W * 8;   // lane stride CH+1: no bank conflicts
     // rows per PSF pass: 8 (two parameters) when they fit in the space the
     // J^T J reduction needs anyway, else 4
-    static constexpr int PAD_ROWS = 8 * PADW * (int)sizeof(T) <= RED_BYTES ? 8 : 4;
-    static constexpr int PAD_BYTES = PAD_ROWS * PADW * (int)sizeof(T);
+    static constexpr int PAD_ROWS = 8 * RS * (int)sizeof(T) <= RED_BYTES ? 8 : 4;
+    static constexpr int PAD_BYTES = PAD_ROWS * RS * (int)sizeof(T);
     static constexpr int EIG_BYTES = (2 * NT * NT + 2 * NT) * 8;
     static constexpr int U0 = PAD_BYTES > EIG_BYTES ? PAD_BYTES : EIG_BYTES;
     // the PSF row buffer, the eigen fallback and the J^T J reduction share
@@ -75,7 +78,8 @@ __device__ __forceinline__ X cld(const X *p)
 // (convolution.c:54-78).  Rows 0 and (when R == 8) 4 are Stokes I rows and
 // use the continuum trick of convolution.c:90-113 with Ic = max(first, last)
 // of that row (mil_sinrf.c:390-395, me_der.c:478-483), taken by in-group
-// shuffles so the row goes to LDS once.  pad: R rows of PADW values, zero
+// shuffles so the row goes to LDS once.  pad: R rows of 2*W*NLPL values
+// (stride one more), zero
 // outside [L/2, L/2 + L).  The FP64 form accumulates in double as
 // convolution.c:68 does; the mixed form in FP32.
 template <typename T, int W, int NLPL, int R, bool IC = true>
@@ -83,7 +87,7 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
                                       T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
-    constexpr int PADW = 2 * W * NLPL;
+    constexpr int RS = 2 * W * NLPL + 1;
     const int half = L / 2;
     const int klast = (L - 1) / W, tlast = (L - 1) % W;
     T ic[2] = {T(0), T(0)};
@@ -105,7 +109,7 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
                 T v = x[r][k];
                 if (IC && r == 0) v = ic[0] - v;
                 if (IC && R == 8 && r == 4) v = ic[1] - v;
-                pad[r * PADW + half + lam] = v;
+                pad[r * RS + half + lam] = v;
             }
         }
     }
@@ -123,7 +127,7 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
             for (int j = jlo; j < jhi; j++) {
                 A gj = A(cld(c->G + j));
 #pragma unroll
-                for (int r = 0; r < R; r++) acc[r] += gj * (A)p0[r * PADW + j];
+                for (int r = 0; r < R; r++) acc[r] += gj * (A)p0[r * RS + j];
             }
 #pragma unroll
             for (int r = 0; r < R; r++) {
@@ -181,7 +185,7 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
 {
     using CF = ConvF<T, W, NLPL, R>;
     typedef typename CF::A A;
-    constexpr int PADW = 2 * W * NLPL, HC = CF::HC, NB = CF::NB, B = CF::B;
+    constexpr int RS = 2 * W * NLPL + 1, HC = CF::HC, NB = CF::NB, B = CF::B;
     const int half = L / 2;
     const int klast = (L - 1) / W, tlast = (L - 1) % W;
     T ic[2] = {T(0), T(0)};
@@ -203,14 +207,14 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
                 T v = x[r][k];
                 if (r == 0) v = ic[0] - v;
                 if (R == 8 && r == 4) v = ic[1] - v;
-                pad[r * PADW + half + lam] = v;
+                pad[r * RS + half + lam] = v;
             }
         }
     }
     wave_sync();
     {
         const int r = t / NB, b = t % NB;
-        const T *src = pad + r * PADW + half - HC + b * B;
+        const T *src = pad + r * RS + half - HC + b * B;
         T w[CF::NWIN];
 #pragma unroll
         for (int i = 0; i < CF::NWIN; i++) w[i] = src[i];
@@ -229,7 +233,7 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
             for (int o = 0; o < B; o++) acc[o] += g[HC + d] * (A)w[o + HC + d];
         const bool irow = (r % 4) == 0;
         const T icr = (R == 8 && r == 4) ? ic[1] : ic[0];
-        T *dst = pad + r * PADW + half + b * B;
+        T *dst = pad + r * RS + half + b * B;
 #pragma unroll
         for (int o = 0; o < B; o++)
             if (b * B + o < L) dst[o] = irow ? T((A)icr - acc[o]) : T(acc[o]);
@@ -240,7 +244,7 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
         int lam = t + k * W;
         if (lam < L)
 #pragma unroll
-            for (int r = 0; r < R; r++) x[r][k] = pad[r * PADW + half + lam];
+            for (int r = 0; r < R; r++) x[r][k] = pad[r * RS + half + lam];
     }
     wave_sync();
 }
@@ -256,14 +260,14 @@ __device__ __forceinline__ void convC(T *pad, const H *h, int jlo, int jhi, int 
                                       T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
-    constexpr int PADW = 2 * W * NLPL;
+    constexpr int RS = 2 * W * NLPL + 1;
     const int half = L / 2;
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         if (lam < L)
 #pragma unroll
-            for (int r = 0; r < R; r++) pad[r * PADW + half + lam] = x[r][k];
+            for (int r = 0; r < R; r++) pad[r * RS + half + lam] = x[r][k];
     }
     wave_sync();
 #pragma unroll
@@ -280,7 +284,7 @@ __device__ __forceinline__ void convC(T *pad, const H *h, int jlo, int jhi, int 
             for (int j = jlo; j < jhi; j++) {
                 A hj = (A)h[j];
 #pragma unroll
-                for (int r = 0; r < R; r++) acc[r] += hj * (A)pad[r * PADW + half + idx];
+                for (int r = 0; r < R; r++) acc[r] += hj * (A)pad[r * RS + half + idx];
                 idx = idx == 0 ? L - 1 : idx - 1;
             }
 #pragma unroll
@@ -612,7 +616,7 @@ pm_invert_kernel(InvArgs a)
     double *red = (double *)pad;     // J^T J reduction (shares the row buffer)
     T *gm = (T *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
     T *gd = gm + W * NLPL;
-    for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
+    for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
     wave_sync();
 
     const int xcd = blockIdx.x % PM_NSLICE;
@@ -719,7 +723,7 @@ pm_invert_kernel(InvArgs a)
 #pragma unroll
                 for (int i = 0; i < NT; i++) delta[i] = ws[2 * NT * NT + NT + i];
                 wave_sync();
-                for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
+                for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
                 wave_sync();
                 if (t == 0) atomicAdd(a.fallbacks, 1ull);
             }
@@ -857,7 +861,7 @@ pm_invert_kernel(InvArgs a)
                 if (part == 0 && e < NV) ab[e] = sum;
                 wave_sync();
             }
-            for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
+            for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
             wave_sync();
             wave_sync();
         }
@@ -904,7 +908,7 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
     const float *slp = (GEN && c->use_slight) ? c->slight : nullptr;
     const int NTR = c->NT;
     const int row9 = (NTR == 10 && slp && !c->fix[9]) ? 10 : 9;
-    for (int i = t; i < LD::PAD_ROWS * LD::PADW; i += W) pad[i] = T(0);
+    for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
     wave_sync();
     const int ngroups = gridDim.x * LD::GPB;
     for (int pix = blockIdx.x * LD::GPB + grp; pix < n; pix += ngroups) {
