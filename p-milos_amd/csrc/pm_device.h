@@ -62,6 +62,10 @@ struct PmConst {
     int use_fft;
 };
 
+// The kernels read the constants through the constant address space: they
+// never write them, and reads at uniform addresses become scalar loads.
+typedef const __attribute__((address_space(4))) PmConst *PmC;
+
 // ------------------------------------------------------------ group helpers
 
 // Cross-lane moves inside a group.  Within a row of 16 lanes DPP
@@ -258,7 +262,7 @@ struct Geom {
 };
 
 template <typename T>
-__device__ __forceinline__ void make_geom(const PmConst *__restrict__ c, const double *m,
+__device__ __forceinline__ void make_geom(PmC c, const double *m,
                                           Geom<T> &g)
 {
     const double CC = PM_PI / 180.0, CC_2 = CC * 2;      // milos.c:232-233
@@ -294,7 +298,7 @@ struct Samp {
 
 // Components and profile sums at one wavelength, mil_sinrf.c:122-183,418-449.
 template <typename T>
-__device__ __forceinline__ void profiles(const PmConst *__restrict__ c, const Geom<T> &g,
+__device__ __forceinline__ void profiles(PmC c, const Geom<T> &g,
                                          double dl, Samp<T> &s)
 {
     // the wavelength offset is formed in double: in FP32 the 0.5 mA spacing
@@ -402,7 +406,7 @@ __device__ __forceinline__ void stokes(const Geom<T> &g, const UR<T> &r, T S[4])
 // convolution, me_der.c:93-312 (d_spectra rows eta0 B vlos dopp aa gm az S0
 // S1).  D[s][p].
 template <typename T, int NT>
-__device__ __forceinline__ void response(const PmConst *__restrict__ c, const Geom<T> &g,
+__device__ __forceinline__ void response(PmC c, const Geom<T> &g,
                                          const Samp<T> &s, const UR<T> &r, T D[4][NT])
 {
     const T LD = g.LD, A = g.A, E0 = g.E0;
@@ -536,10 +540,10 @@ __device__ __forceinline__ void response(const PmConst *__restrict__ c, const Ge
 // ------------------------------------------------------------ model update
 
 // AplicaDelta, milosUtils.c:47-115.
-__device__ __forceinline__ void aplica_delta(const PmConst *__restrict__ c, int NT,
+__device__ __forceinline__ void aplica_delta(PmC c, int NT,
                                              const double *cur, double *delta, double *out)
 {
-    const int *f = c->fix;
+    const auto *f = c->fix;
     if (f[0]) out[0] = cur[0] - delta[0];
     if (f[1]) {
         if (delta[1] < -300) delta[1] = -300;
@@ -782,7 +786,7 @@ struct Classic {
 // (milos.c:976-983); without it (the estimates-only mode, milos.c:493) B_los
 // is returned for the S0 slot (:449-450) and nothing is guarded.  Not
 // inlined: it runs once per pixel.
-__device__ __noinline__ Classic classical(const PmConst *__restrict__ c, const float *spec,
+__device__ __noinline__ Classic classical(PmC c, const float *spec,
                                          bool init)
 {
     const int n = c->L;

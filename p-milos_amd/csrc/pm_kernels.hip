@@ -65,15 +65,6 @@ struct Lds {
 
 // ------------------------------------------------------------ convolution
 
-// A uniform read of the constants through the constant address space, so
-// that it is a scalar load (the constants are not written while a kernel
-// runs).
-template <typename X>
-__device__ __forceinline__ X cld(const X *p)
-{
-    return *(const __attribute__((address_space(4))) X *)p;
-}
-
 // Zero-padded "same" correlation of R rows x[r][slot] with the PSF, in place
 // (convolution.c:54-78).  Rows 0 and (when R == 8) 4 are Stokes I rows and
 // use the continuum trick of convolution.c:90-113 with Ic = max(first, last)
@@ -83,7 +74,7 @@ __device__ __forceinline__ X cld(const X *p)
 // outside [L/2, L/2 + L).  The FP64 form accumulates in double as
 // convolution.c:68 does; the mixed form in FP32.
 template <typename T, int W, int NLPL, int R, bool IC = true>
-__device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int t, int L,
+__device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
                                       T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
@@ -125,7 +116,7 @@ __device__ __forceinline__ void convR(const PmConst *__restrict__ c, T *pad, int
             for (int r = 0; r < R; r++) acc[r] = 0;
             const T *p0 = pad + lam;
             for (int j = jlo; j < jhi; j++) {
-                A gj = A(cld(c->G + j));
+                A gj = A(c->G[j]);
 #pragma unroll
                 for (int r = 0; r < R; r++) acc[r] += gj * (A)p0[r * RS + j];
             }
@@ -172,7 +163,7 @@ struct ConvF {
 };
 
 template <typename T, int W, int NLPL, int R>
-__device__ __forceinline__ bool convF_usable(const PmConst *__restrict__ c, int L)
+__device__ __forceinline__ bool convF_usable(PmC c, int L)
 {
     using CF = ConvF<T, W, NLPL, R>;
     const int reach = sizeof(T) == 8 ? c->reach64 : c->reach32;
@@ -180,7 +171,7 @@ __device__ __forceinline__ bool convF_usable(const PmConst *__restrict__ c, int 
 }
 
 template <typename T, int W, int NLPL, int R>
-__device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int t, int L,
+__device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
                                       T x[R][NLPL])
 {
     using CF = ConvF<T, W, NLPL, R>;
@@ -221,12 +212,12 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
         A acc[B];
 #pragma unroll
         for (int o = 0; o < B; o++) acc[o] = 0;
-        const T *gt = sizeof(T) == 8 ? (const T *)c->gc64 : (const T *)c->gc32;
         // every tap of the window; those beyond the PSF's tap range are 0
         // and add exactly 0
         A g[2 * HC + 1];
 #pragma unroll
-        for (int d = -HC; d <= HC; d++) g[HC + d] = (A)cld(gt + PM_HC + d);
+        for (int d = -HC; d <= HC; d++)
+            g[HC + d] = sizeof(T) == 8 ? (A)c->gc64[PM_HC + d] : (A)c->gc32[PM_HC + d];
 #pragma unroll
         for (int d = -HC; d <= HC; d++)
 #pragma unroll
@@ -255,8 +246,8 @@ __device__ __forceinline__ void convF(const PmConst *__restrict__ c, T *pad, int
 // the kernel's taps [jlo, jhi): the others are below the rounding of the
 // sum (see Taps).  The rows are staged in the middle of the PSF row buffer,
 // leaving its zero margins untouched.
-template <typename T, int W, int NLPL, int R, typename H = T>
-__device__ __forceinline__ void convC(T *pad, const H *h, int jlo, int jhi, int t, int L,
+template <typename T, int W, int NLPL, int R, typename HP>
+__device__ __forceinline__ void convC(T *pad, HP h, int jlo, int jhi, int t, int L,
                                       T x[R][NLPL])
 {
     typedef typename AccT<T>::type A;
@@ -300,13 +291,13 @@ __device__ __forceinline__ void convC(T *pad, const H *h, int jlo, int jhi, int 
 // PSF's transform and the -L/2 rotation is convCircular with the PSF as
 // kernel).  The FFT mode is handled by the general form only.
 template <typename T, int W, int NLPL, int R, bool GEN, bool IC = true>
-__device__ __forceinline__ void convPSF(const PmConst *__restrict__ c, T *pad, int t, int L,
+__device__ __forceinline__ void convPSF(PmC c, T *pad, int t, int L,
                                         T x[R][NLPL])
 {
     if (GEN && c->use_fft) {
         const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
         const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
-        convC<T, W, NLPL, R, double>(pad, c->G, jlo, jhi, t, L, x);
+        convC<T, W, NLPL, R>(pad, c->G, jlo, jhi, t, L, x);
     } else if (!GEN && IC && ConvF<T, W, NLPL, R>::ok && convF_usable<T, W, NLPL, R>(c, L)) {
         convF<T, W, NLPL, R>(c, pad, t, L, x);
     } else {
@@ -347,7 +338,7 @@ __device__ __forceinline__ void tap_window(const T v[NLPL], int t, int L, int &l
 // or circularly convolved with it in the FFT mode (mil_sinrf.c:302-314,
 // me_der.c:331-354); written to gm / gd in LDS, with their tap windows.
 template <typename T, int W, int NLPL>
-__device__ __forceinline__ void mac_kernels(const PmConst *__restrict__ c, const double *m,
+__device__ __forceinline__ void mac_kernels(PmC c, const double *m,
                                             T *pad, int t, int L, T *gm, T *gd, Taps &tp)
 {
     const double MC = m[9];
@@ -397,7 +388,7 @@ struct Extra {
 
 // Zero the response-function row of a fixed parameter,
 // FijaACeroDerivadasNoNecesarias, milosUtils.c:199-227.
-__device__ __forceinline__ bool row_is_fixed(const PmConst *__restrict__ c, int NT, int p)
+__device__ __forceinline__ bool row_is_fixed(PmC c, int NT, int p)
 {
     if (NT == 9 || NT == 11) return c->fix[p] == 0;
     if (p < 9) return c->fix[p] == 0;
@@ -409,7 +400,7 @@ __device__ __forceinline__ bool row_is_fixed(const PmConst *__restrict__ c, int 
 // (mil_sinrf.c:287-403) and mixed with the stray light (:405-412).  The
 // general form (GEN) also handles macroturbulence and stray light.
 template <typename T, int W, int NLPL, bool GEN>
-__device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const double *m,
+__device__ __forceinline__ void synth_slots(PmC c, const double *m,
                                             const float *slp, T *pad, T *gm, T *gd, int t,
                                             int L, Geom<T> &g, Samp<T> sp[NLPL], T S[4][NLPL],
                                             Extra<T, NLPL> &ex)
@@ -465,7 +456,7 @@ __device__ __forceinline__ void synth_slots(const PmConst *__restrict__ c, const
 // the first call (milosUtils.c:537), the mixed one after an accepted step
 // (:587), selected by ff_pre.
 template <typename T, int W, int NLPL, int NT, bool GEN>
-__device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const double *m,
+__device__ __forceinline__ void rf_slots(PmC c, const double *m,
                                          const float *slp, const Geom<T> &g,
                                          const Samp<T> sp[NLPL], T *pad, const T *gm,
                                          const T *gd, int t, int L, const Extra<T, NLPL> &ex,
@@ -481,7 +472,11 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
     // form (its register-blocked windows stay small)
     constexpr int R = (!GEN && sizeof(T) == 4) ? 4 : Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
     constexpr int PPC = R / 4;
-    constexpr int NPS = 8;                      // rows 0..8 but S0's
+    // rows 0..8 but S0's; the 9-row form also leaves S1's to the spectrum
+    // (below), the general one convolves it (stray light scales it)
+    constexpr bool S1_FROM_S = !GEN;
+    constexpr int NPS = S1_FROM_S ? 7 : 8;
+    constexpr int NFULL = NPS / PPC;
     const bool mac = GEN && ex.mac;
     if (mac && NT >= 10) {                      // v_mac row from the raw synthesis
         T x[4][NLPL];
@@ -497,12 +492,11 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
     }
     if (mac || c->use_psf) {
 #pragma unroll
-        for (int q0 = 0; q0 < NPS; q0 += PPC) {
+        for (int q0 = 0; q0 < NFULL * PPC; q0 += PPC) {
             T x[R][NLPL];
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
-                int q = q0 + u < NPS ? q0 + u : NPS - 1;
-                int p = q < 7 ? q : q + 1;
+                const int q = q0 + u, p = q < 7 ? q : q + 1;
 #pragma unroll
                 for (int k = 0; k < NLPL; k++)
 #pragma unroll
@@ -512,14 +506,38 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
             else convPSF<T, W, NLPL, R, GEN>(c, pad, t, L, x);
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
-                if (q0 + u >= NPS) continue;
-                int q = q0 + u;
-                int p = q < 7 ? q : q + 1;
+                const int q = q0 + u, p = q < 7 ? q : q + 1;
 #pragma unroll
                 for (int k = 0; k < NLPL; k++)
 #pragma unroll
                     for (int s = 0; s < 4; s++) D[k][s][p] = x[4 * u + s][k];
             }
+        }
+        if (NPS % PPC) {                        // the last parameter alone
+            constexpr int q = NFULL * PPC, p = q < 7 ? q : q + 1;
+            T x[4][NLPL];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) x[s][k] = D[k][s][p];
+            if (mac) convC<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
+            else convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, x);
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) D[k][s][p] = x[s][k];
+        }
+        if (S1_FROM_S) {
+            // the S1 row is (I - S0)/S1 on I and Q/S1, U/S1, V/S1 before the
+            // PSF, and the PSF maps it (the continuum trick included) to the
+            // same expressions of the convolved spectrum S of this model
+            typedef typename AccT<T>::type A;
+            const A s0 = (A)m[7], is1 = A(1) / (A)m[8];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    D[k][s][8] = T(((A)S[s][k] - (s == 0 ? s0 : A(0))) * is1);
         }
     }
     if (GEN && slp) {
@@ -538,6 +556,11 @@ __device__ __forceinline__ void rf_slots(const PmConst *__restrict__ c, const do
                 }
         }
     }
+    // the azimuth rows of I and V are 0 (me_der.c:309-312) and stay 0 through
+    // the convolutions and the stray-light scaling; restated so that their
+    // J^T J terms drop out
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) { D[k][0][6] = T(0); D[k][3][6] = T(0); }
 }
 
 // ------------------------------------------------------------ work queue
@@ -603,7 +626,7 @@ pm_invert_kernel(InvArgs a)
     using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
     constexpr int NP = LD::NP;
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
-    const PmConst *__restrict__ c = a.c;
+    PmC c = (PmC)a.c;
     const int L = c->L;
     const int grp = threadIdx.x / W, t = threadIdx.x % W;
     char *gb = smem + grp * LD::GROUP_BYTES;
@@ -650,7 +673,7 @@ pm_invert_kernel(InvArgs a)
             const float *src = a.stokes + (size_t)pix * 4 * L;
             if (GEN)                                          // milos.c:986-996
                 slp = a.slight ? a.slight + (size_t)pix * 4 * L
-                               : (c->use_slight ? c->slight : nullptr);
+                               : (c->use_slight ? (const float *)c->slight : nullptr);
             int ng = 0;
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
@@ -895,8 +918,9 @@ pm_invert_kernel(InvArgs a)
 // the filling factor when me_der puts it there, me_der.c:505-520).
 template <typename T, int W, int NLPL, int NT, bool GEN>
 __global__ void __launch_bounds__(BlockOf<W>::value)
-pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, double *rf)
+pm_synth_kernel(const PmConst *c0, const double *models, int n, double *spectra, double *rf)
 {
+    PmC c = (PmC)c0;
     using LD = Lds<T, W, NLPL, NT, GEN>;
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
     const int L = c->L;
@@ -905,7 +929,7 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
     T *pad = (T *)gb;
     T *gm = (T *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
     T *gd = gm + W * NLPL;
-    const float *slp = (GEN && c->use_slight) ? c->slight : nullptr;
+    const float *slp = (GEN && c->use_slight) ? (const float *)c->slight : nullptr;
     const int NTR = c->NT;
     const int row9 = (NTR == 10 && slp && !c->fix[9]) ? 10 : 9;
     for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
@@ -960,8 +984,9 @@ pm_synth_kernel(const PmConst *c, const double *models, int n, double *spectra, 
 // The classical-estimates mode (NumberOfCycles < 0, milos.c:474-500): the
 // guess with B, vlos, gamma, phi from estimacionesClasicas(forInitialUse = 0)
 // and B_los in the S0 slot; one thread per pixel.
-__global__ void pm_classical_kernel(const PmConst *c, const float *stokes, int n, double *models)
+__global__ void pm_classical_kernel(const PmConst *c0, const float *stokes, int n, double *models)
 {
+    PmC c = (PmC)c0;
     const int pix = blockIdx.x * blockDim.x + threadIdx.x;
     if (pix >= n) return;
     const Classic e = classical(c, stokes + (size_t)pix * 4 * c->L, false);
@@ -1257,24 +1282,30 @@ struct InvVariant {
 template <typename T, int NT, bool G>
 static int variants_of(int L, InvVariant *v)
 {
+    // Several samples per lane only in the FP32 form: the FP64 kernels of
+    // those shapes need far more than 256 registers per lane (occupancy 1,
+    // never the fastest), and twice their inversions came out wrong on the
+    // GPU after changes elsewhere in the kernel while the same source gave
+    // right results in every other shape.  L > 64 needs them in FP64 too.
+    constexpr bool MULTI = !G && sizeof(T) == 4;
     int n = 0;
     if (L <= 8) {
         v[n++] = PM_V(T, 8, 1, NT, G, 128);
     } else if (L <= 16) {
         v[n++] = PM_V(T, 16, 1, NT, G, 256);
         v[n++] = PM_V(T, 16, 1, NT, G, 128);
-        if constexpr (!G) v[n++] = PM_V(T, 8, 2, NT, G, 128);
+        if constexpr (MULTI) v[n++] = PM_V(T, 8, 2, NT, G, 128);
     } else if (L <= 32) {
         v[n++] = PM_V(T, 32, 1, NT, G, 256);
         v[n++] = PM_V(T, 32, 1, NT, G, 128);
-        if constexpr (!G) {
+        if constexpr (MULTI) {
             v[n++] = PM_V(T, 16, 2, NT, G, 256);
             v[n++] = PM_V(T, 16, 2, NT, G, 128);
         }
     } else if (L <= 64) {
         v[n++] = PM_V(T, 64, 1, NT, G, 256);
         v[n++] = PM_V(T, 64, 1, NT, G, 128);
-        if constexpr (!G) v[n++] = PM_V(T, 32, 2, NT, G, 256);
+        if constexpr (MULTI) v[n++] = PM_V(T, 32, 2, NT, G, 256);
     } else {
         v[n++] = PM_V(T, 64, 2, NT, G, 256);
         v[n++] = PM_V(T, 64, 2, NT, G, 128);
