@@ -614,9 +614,12 @@ __device__ __forceinline__ void check_model(double *m)
 // cleared, so its component comes out 0; that is done up front (rarely
 // needed), and the factorisation itself forms each row's products L_jk d_k
 // once for its pivot and the rows below.
+// check = false skips the eigenvalue bound: the caller has already passed it
+// for the same alpha at a damping no larger (alpha + lambda diag(alpha)
+// only grows with lambda, so trace(A^-1) only falls).
 template <int N>
 __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower alpha, then beta */,
-                                          double damp, double *x)
+                                          double damp, double *x, bool check = true)
 {
     constexpr int P = N * (N + 1) / 2;
     double F[P], id[N];
@@ -680,6 +683,7 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         for (int k = i + 1; k < N; k++) v -= F[k * (k + 1) / 2 + i] * x[k];
         x[i] = act[i] ? v : 0.0;
     }
+    if (!check) return true;
     // L^-1 in place (column by column), then trace(A^-1) =
     // sum_k (1/d_k) sum_{i<=k} (L^-1)_{ki}^2 over the active rows; 1/trace
     // bounds the smallest eigenvalue from below

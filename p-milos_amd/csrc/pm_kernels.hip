@@ -646,6 +646,9 @@ pm_invert_kernel(InvArgs a)
     int sl_off = 0, cur = 0, end = 0;
 
     double flambda = 0, ochisqr = 0, chisqr0 = 0, factor = 1.0;
+    // damping at which the current alpha last passed the eigenvalue bound
+    // of the solve (negative: not yet)
+    double chk_lambda = -1.0;
     int it = 0, nacc = 0, nfree = 1, pix = -1;
     bool iterating = false;
     // the observed samples stay in LDS (sobs) for the pixel's whole LM loop
@@ -731,7 +734,9 @@ pm_invert_kernel(InvArgs a)
         } else {
             // covar = alpha with the diagonal scaled by (1 + lambda), :559-563
             double delta[NT];
-            bool ok = ldl_solve<NT>(ab, 1.0 + flambda, delta);
+            const bool check = !(chk_lambda >= 0.0 && flambda >= chk_lambda);
+            bool ok = ldl_solve<NT>(ab, 1.0 + flambda, delta, check);
+            if (ok && check) chk_lambda = flambda;
             if (!ok) {
                 double *ws = (double *)pad;
                 for (int e = t; e < NT * NT; e += W) {
@@ -812,6 +817,7 @@ pm_invert_kernel(InvArgs a)
         }
 
         if (do_rf) {
+            chk_lambda = -1.0;
             T D[NLPL][4][NT];
             rf_slots<T, W, NLPL, NT, GEN>(c, lm, slp, g, sp, pad, gm, gd, t, L, ex, first, S, D);
             // covarm2 (lib.c:71-116): each alpha/beta entry is a per-lane
