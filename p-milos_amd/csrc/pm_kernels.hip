@@ -170,7 +170,7 @@ __device__ __forceinline__ bool convF_usable(PmC c, int L)
     return CF::ok && reach <= CF::HC && CF::HC <= L / 2;
 }
 
-template <typename T, int W, int NLPL, int R>
+template <typename T, int W, int NLPL, int R, bool IC = true>
 __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
                                       T x[R][NLPL])
 {
@@ -181,7 +181,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
     const int klast = (L - 1) / W, tlast = (L - 1) % W;
     T ic[2] = {T(0), T(0)};
 #pragma unroll
-    for (int q = 0; q < (R == 8 ? 2 : 1); q++) {
+    for (int q = 0; q < (IC ? (R == 8 ? 2 : 1) : 0); q++) {
         const int r = 4 * q;
         T last = x[r][0];
 #pragma unroll
@@ -196,8 +196,8 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
 #pragma unroll
             for (int r = 0; r < R; r++) {
                 T v = x[r][k];
-                if (r == 0) v = ic[0] - v;
-                if (R == 8 && r == 4) v = ic[1] - v;
+                if (IC && r == 0) v = ic[0] - v;
+                if (IC && R == 8 && r == 4) v = ic[1] - v;
                 pad[r * RS + half + lam] = v;
             }
         }
@@ -222,7 +222,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
         for (int d = -HC; d <= HC; d++)
 #pragma unroll
             for (int o = 0; o < B; o++) acc[o] += g[HC + d] * (A)w[o + HC + d];
-        const bool irow = (r % 4) == 0;
+        const bool irow = IC && (r % 4) == 0;
         const T icr = (R == 8 && r == 4) ? ic[1] : ic[0];
         T *dst = pad + r * RS + half + b * B;
 #pragma unroll
@@ -285,11 +285,120 @@ __device__ __forceinline__ void convC(T *pad, HP h, int jlo, int jhi, int t, int
     wave_sync();
 }
 
+// The circular convolution of convC, register-blocked: the rows are stored
+// twice (sample m at TC + ((m - L/2) mod L) and L further on), so that the
+// samples of output m under tap j sit at TC + m - j + L and each lane's
+// block of B consecutive outputs reads one contiguous window per chunk of
+// TC taps.  The taps come from the pixel's kernel in LDS, in ascending order
+// as in convC (chunks padded with zero taps), so the results are the same.
+// Every lane loads its windows before any lane stores; the doubled rows are
+// cleared afterwards (the other convolutions expect zero margins).
+template <typename T, int W, int NLPL, int R>
+struct ConvCF {
+    typedef typename AccT<T>::type A;
+    static constexpr int NX = W * NLPL;
+    static constexpr int NB = W / R;
+    static constexpr int B = NX / NB;
+    static constexpr int TC = 8;
+    static constexpr int NWIN = B + TC - 1;
+    static constexpr bool ok = W % R == 0 &&
+        (NWIN * (int)sizeof(T) + B * (int)sizeof(A)) / 4 <= 64;
+};
+
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ bool convCF_usable(int L)
+{
+    // every window stays inside its row of 2*W*NLPL + 1
+    return ConvCF<T, W, NLPL, R>::ok && L + ConvCF<T, W, NLPL, R>::TC - 1 <= W * NLPL;
+}
+
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int t, int L,
+                                       T x[R][NLPL])
+{
+    using CF = ConvCF<T, W, NLPL, R>;
+    typedef typename CF::A A;
+    constexpr int RS = 2 * W * NLPL + 1, NB = CF::NB, B = CF::B, TC = CF::TC;
+    const int half = L / 2;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        const int lam = t + k * W;
+        if (lam < L) {
+            int q = lam - half;
+            if (q < 0) q += L;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                pad[r * RS + TC + q] = x[r][k];
+                pad[r * RS + TC + q + L] = x[r][k];
+            }
+        }
+    }
+    wave_sync();
+    const int r = t / NB, b = t % NB, m0 = b * B;
+    A acc[B];
+#pragma unroll
+    for (int o = 0; o < B; o++) acc[o] = 0;
+    if (m0 < L) {
+        const T *row = pad + r * RS + m0 + L + 1;
+        for (int j0 = jlo; j0 < jhi; j0 += TC) {
+            const T *src = row - j0;
+            T w[CF::NWIN];
+#pragma unroll
+            for (int i = 0; i < CF::NWIN; i++) w[i] = src[i];
+#pragma unroll
+            for (int u = 0; u < TC; u++) {
+                const int j = j0 + u;
+                const A hj = j < jhi ? (A)h[j < jhi ? j : jhi - 1] : A(0);
+#pragma unroll
+                for (int o = 0; o < B; o++) acc[o] += hj * (A)w[o + TC - 1 - u];
+            }
+        }
+    }
+    wave_sync();
+    T *dst = pad + r * RS + TC;
+#pragma unroll
+    for (int o = 0; o < B; o++)
+        if (m0 + o < L) dst[m0 + o] = T(acc[o]);
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        const int lam = t + k * W;
+        if (lam < L)
+#pragma unroll
+            for (int r2 = 0; r2 < R; r2++) x[r2][k] = pad[r2 * RS + TC + lam];
+    }
+    wave_sync();
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        const int lam = t + k * W;
+        if (lam < L) {
+            int q = lam - half;
+            if (q < 0) q += L;
+#pragma unroll
+            for (int r2 = 0; r2 < R; r2++) {
+                pad[r2 * RS + TC + q] = T(0);
+                pad[r2 * RS + TC + q + L] = T(0);
+            }
+        }
+    }
+    wave_sync();
+}
+
+// The macroturbulence convolutions: register-blocked where the rows fit.
+template <typename T, int W, int NLPL, int R>
+__device__ __forceinline__ void convMac(T *pad, const T *h, int jlo, int jhi, int t, int L,
+                                       T x[R][NLPL])
+{
+    if (convCF_usable<T, W, NLPL, R>(L)) convCF<T, W, NLPL, R>(pad, h, jlo, jhi, t, L, x);
+    else convC<T, W, NLPL, R>(pad, h, jlo, jhi, t, L, x);
+}
+
 // The PSF applied to R rows: zero-padded correlation with the continuum
 // trick on I (direct mode), or circular convolution of every row (FFT
 // mode, mil_sinrf.c:362-386, me_der.c:441-470: the DFT product with the
 // PSF's transform and the -L/2 rotation is convCircular with the PSF as
-// kernel).  The FFT mode is handled by the general form only.
+// kernel).  The FFT mode is handled by the general form only.  The direct
+// correlation is register-blocked where the PSF's taps fit (convF).
 template <typename T, int W, int NLPL, int R, bool GEN, bool IC = true>
 __device__ __forceinline__ void convPSF(PmC c, T *pad, int t, int L,
                                         T x[R][NLPL])
@@ -298,8 +407,8 @@ __device__ __forceinline__ void convPSF(PmC c, T *pad, int t, int L,
         const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
         const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
         convC<T, W, NLPL, R>(pad, c->G, jlo, jhi, t, L, x);
-    } else if (!GEN && IC && ConvF<T, W, NLPL, R>::ok && convF_usable<T, W, NLPL, R>(c, L)) {
-        convF<T, W, NLPL, R>(c, pad, t, L, x);
+    } else if (ConvF<T, W, NLPL, R>::ok && convF_usable<T, W, NLPL, R>(c, L)) {
+        convF<T, W, NLPL, R, IC>(c, pad, t, L, x);
     } else {
         convR<T, W, NLPL, R, IC>(c, pad, t, L, x);
     }
@@ -428,7 +537,7 @@ __device__ __forceinline__ void synth_slots(PmC c, const double *m,
     }
     if (ex.mac) {
         mac_kernels<T, W, NLPL>(c, m, pad, t, L, gm, gd, ex.tp);
-        convC<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, S);   // :353-354
+        convMac<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, S); // :353-354
     } else if (c->use_psf) {
         convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, S);              // :362-402
     }
@@ -484,7 +593,7 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
         for (int k = 0; k < NLPL; k++)
 #pragma unroll
             for (int s = 0; s < 4; s++) x[s][k] = ex.Sraw[s][k];
-        convC<T, W, NLPL, 4>(pad, gd, ex.tp.dlo, ex.tp.dhi, t, L, x);
+        convMac<T, W, NLPL, 4>(pad, gd, ex.tp.dlo, ex.tp.dhi, t, L, x);
 #pragma unroll
         for (int k = 0; k < NLPL; k++)
 #pragma unroll
@@ -502,7 +611,7 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
 #pragma unroll
                     for (int s = 0; s < 4; s++) x[4 * u + s][k] = D[k][s][p];
             }
-            if (mac) convC<T, W, NLPL, R>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
+            if (mac) convMac<T, W, NLPL, R>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
             else convPSF<T, W, NLPL, R, GEN>(c, pad, t, L, x);
 #pragma unroll
             for (int u = 0; u < PPC; u++) {
@@ -520,7 +629,7 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
             for (int k = 0; k < NLPL; k++)
 #pragma unroll
                 for (int s = 0; s < 4; s++) x[s][k] = D[k][s][p];
-            if (mac) convC<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
+            if (mac) convMac<T, W, NLPL, 4>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
             else convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, x);
 #pragma unroll
             for (int k = 0; k < NLPL; k++)
