@@ -942,7 +942,11 @@ pm_invert_kernel(InvArgs a)
                 }
             }
             typedef typename AccT<T>::type A;
-            A wsig[NLPL][4], rs[NLPL][4];
+            // the weights w/sigma^2 are folded into the rows once, as their
+            // square roots, so that every J^T J term is one product; the
+            // gradient takes sqrt(w/sigma^2)(S - obs).  The literal zeros of
+            // the azimuth I/V rows and of the S0 row off I stay literal.
+            A rs[NLPL][4];
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
                 const int lam = t + k * W;
@@ -950,9 +954,13 @@ pm_invert_kernel(InvArgs a)
                 for (int s = 0; s < 4; s++) {
                     const float o = lam < L ? sobs[s * L + lam] : 0.0f;
                     double isg = o <= -1.0f ? 1e-21 : c->inv_noise;
-                    double w = c->weight[s];
-                    wsig[k][s] = A(w * isg);
-                    rs[k][s] = A((w * ((double)S[s][k] - (double)o)) * isg);
+                    const A sw = A(sqrt(c->weight[s] * isg));
+                    rs[k][s] = sw * A((double)S[s][k] - (double)o);
+#pragma unroll
+                    for (int p = 0; p < NT; p++) {
+                        if ((p == 6 && (s == 0 || s == 3)) || (p == 7 && s != 0)) continue;
+                        D[k][s][p] = T((A)D[k][s][p] * sw);
+                    }
                 }
             }
             // The NP + NT sums are reduced CH at a time through LDS: every lane
@@ -974,7 +982,7 @@ pm_invert_kernel(InvArgs a)
                         for (int k = 0; k < NLPL; k++)
 #pragma unroll
                             for (int s = 0; s < 4; s++)
-                                v += ((A)D[k][s][i] * (A)D[k][s][h]) * wsig[k][s];
+                                v += (A)D[k][s][i] * (A)D[k][s][h];
                         red[t * (CH + 1) + idx % CH] = (double)v;
                     }
                     const int ib = NP + i;
