@@ -655,7 +655,11 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         }
         if (!(dj > 0.0)) return false;
         F[jj + j] = dj;
-        const double inv_dj = 1.0 / dj;
+        // 1/d_j from the hardware reciprocal and two Newton steps (a pivot
+        // scale, not a reference quantity: it only has to be accurate)
+        double inv_dj = __builtin_amdgcn_rcp(dj);
+        inv_dj = fma(fma(-dj, inv_dj, 1.0), inv_dj, inv_dj);
+        inv_dj = fma(fma(-dj, inv_dj, 1.0), inv_dj, inv_dj);
         id[j] = inv_dj;
 #pragma unroll
         for (int i = j + 1; i < N; i++) {

@@ -998,9 +998,15 @@ pm_invert_kernel(InvArgs a)
                 wave_sync();
                 const int vv = t % CH, part = t / CH;
                 constexpr int SPAN = W / PARTS;
-                double sum = 0;
+                // pairwise, so the adds form a tree of depth log2(SPAN)
+                double v[SPAN];
 #pragma unroll
-                for (int l = 0; l < SPAN; l++) sum += red[(part * SPAN + l) * (CH + 1) + vv];
+                for (int l = 0; l < SPAN; l++) v[l] = red[(part * SPAN + l) * (CH + 1) + vv];
+#pragma unroll
+                for (int w = 1; w < SPAN; w *= 2)
+#pragma unroll
+                    for (int l = 0; l + w < SPAN; l += 2 * w) v[l] += v[l + w];
+                double sum = v[0];
                 if (PARTS >= 2) sum += swz16_d(sum);
                 if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
                 const int e = ch * CH + vv;
