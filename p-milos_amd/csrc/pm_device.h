@@ -149,8 +149,15 @@ __device__ __forceinline__ double group_sum_comp(float v)
     return r;
 }
 
-// 1/x: IEEE division in FP64; the hardware reciprocal (1 ulp) in FP32
-__device__ __forceinline__ double recip(double x) { return 1.0 / x; }
+// 1/x: in FP64 the hardware reciprocal refined by two Newton steps (within
+// about an ulp of the division; the arguments here are never 0 or infinite);
+// the hardware reciprocal (1 ulp) in FP32
+__device__ __forceinline__ double recip(double x)
+{
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(fma(-x, r, 1.0), r, r);
+    return fma(fma(-x, r, 1.0), r, r);
+}
 __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x); }
 
 template <typename T> struct AccT { typedef double type; };
@@ -232,7 +239,7 @@ __device__ __forceinline__ void voigt(T a, T v, T &H, T &F)
     // where |d|^2 would overflow for |v| beyond ~2000
     T wr, wi;
     if (sizeof(T) == 8) {
-        T inv = T(1) / (dr * dr + di * di);
+        T inv = recip(dr * dr + di * di);
         wr = (nr * dr + ni * di) * inv;
         wi = (ni * dr - nr * di) * inv;
     } else if (fabs(dr) >= fabs(di)) {
@@ -655,11 +662,7 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
         }
         if (!(dj > 0.0)) return false;
         F[jj + j] = dj;
-        // 1/d_j from the hardware reciprocal and two Newton steps (a pivot
-        // scale, not a reference quantity: it only has to be accurate)
-        double inv_dj = __builtin_amdgcn_rcp(dj);
-        inv_dj = fma(fma(-dj, inv_dj, 1.0), inv_dj, inv_dj);
-        inv_dj = fma(fma(-dj, inv_dj, 1.0), inv_dj, inv_dj);
+        const double inv_dj = recip(dj);
         id[j] = inv_dj;
 #pragma unroll
         for (int i = j + 1; i < N; i++) {
