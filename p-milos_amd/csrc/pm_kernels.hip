@@ -419,6 +419,7 @@ __device__ __forceinline__ void convPSF(PmC c, T *pad, int t, int L,
 // kernel's largest; the rest are below the rounding of the sums they enter.
 struct Taps {
     int mlo, mhi, dlo, dhi;
+    double msum;         // sum of the kernel's taps in [mlo, mhi)
 };
 
 template <typename T, int W, int NLPL>
@@ -483,6 +484,13 @@ __device__ __forceinline__ void mac_kernels(PmC c, const double *m,
     }
     tap_window<T, W, NLPL>(x[0], t, L, tp.mlo, tp.mhi);
     tap_window<T, W, NLPL>(x[1], t, L, tp.dlo, tp.dhi);
+    double tsum = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        int lam = t + k * W;
+        if (lam >= tp.mlo && lam < tp.mhi) tsum += (double)x[0][k];
+    }
+    tp.msum = group_sum<W>(tsum);
     wave_sync();
 }
 
@@ -581,10 +589,9 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
     // form (its register-blocked windows stay small)
     constexpr int R = (!GEN && sizeof(T) == 4) ? 4 : Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
     constexpr int PPC = R / 4;
-    // rows 0..8 but S0's; the 9-row form also leaves S1's to the spectrum
-    // (below), the general one convolves it (stray light scales it)
-    constexpr bool S1_FROM_S = !GEN;
-    constexpr int NPS = S1_FROM_S ? 7 : 8;
+    // rows 0..8 but S0's and S1's: S1's comes from the convolved spectrum
+    // (below), or, in the FFT mode without macroturbulence, a pass of its own
+    constexpr int NPS = 7;
     constexpr int NFULL = NPS / PPC;
     const bool mac = GEN && ex.mac;
     if (mac && NT >= 10) {                      // v_mac row from the raw synthesis
@@ -636,17 +643,33 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
 #pragma unroll
                 for (int s = 0; s < 4; s++) D[k][s][p] = x[s][k];
         }
-        if (S1_FROM_S) {
-            // the S1 row is (I - S0)/S1 on I and Q/S1, U/S1, V/S1 before the
-            // PSF, and the PSF maps it (the continuum trick included) to the
-            // same expressions of the convolved spectrum S of this model
-            typedef typename AccT<T>::type A;
-            const A s0 = (A)m[7], is1 = A(1) / (A)m[8];
+        if (GEN && !mac && c->use_fft) {
+            T x[4][NLPL];
 #pragma unroll
             for (int k = 0; k < NLPL; k++)
 #pragma unroll
-                for (int s = 0; s < 4; s++)
-                    D[k][s][8] = T(((A)S[s][k] - (s == 0 ? s0 : A(0))) * is1);
+                for (int s = 0; s < 4; s++) x[s][k] = D[k][s][8];
+            convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, x);
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) D[k][s][8] = x[s][k];
+        } else {
+            // the S1 row is (I - S0)/S1 on I and Q/S1, U/S1, V/S1 before the
+            // convolution, which maps it to the same expressions of the
+            // convolved spectrum of this model (before any stray-light mix):
+            // with the PSF alone through the continuum trick, with
+            // macroturbulence through the circular sum of the taps, which
+            // takes a constant c to c times their sum
+            typedef typename AccT<T>::type A;
+            const A s0 = (A)(m[7] * (mac ? ex.tp.msum : 1.0)), is1 = A(1) / (A)m[8];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++)
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    const T sc = (GEN && slp) ? ex.Ssl[s][k] : S[s][k];
+                    D[k][s][8] = T(((A)sc - (s == 0 ? s0 : A(0))) * is1);
+                }
         }
     }
     if (GEN && slp) {
