@@ -887,3 +887,99 @@ __device__ __noinline__ Classic classical(PmC c, const float *spec,
     }
     return r;
 }
+
+// The same estimates computed by a pixel group: each lane takes its own
+// samples (lambda = t + k*W) and the sums over wavelength are group sums
+// (tree order instead of the serial order), the range of the leading and
+// trailing negative I a group min/max.  Every lane returns the same result.
+template <int W, int NLPL>
+__device__ __noinline__ Classic classical_group(PmC c, const float *spec, bool init, int t)
+{
+    const int n = c->L;
+    const double l0 = c->lambda0;
+    const float *sI = spec, *sQ = spec + n, *sU = spec + 2 * n, *sV = spec + 3 * n;
+    // milosUtils.c:301-325: begin = first sample with I not < 0, end = one
+    // past the last
+    int lo = n, hi = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        const int i = t + k * W;
+        if (i < n && !(sI[i] < 0)) { lo = min(lo, i); hi = max(hi, i + 1); }
+    }
+    const int begin = group_min_int<W>(lo), end = group_max_int<W>(hi);
+    const double Ic = (end >= 1) ? (double)spec[end - 1] : 0.0;
+    double x = 0, y = 0, xv = 0, yv = 0, xm = 0, ym = 0, xg = 0, yg = 0, su = 0, sq = 0;
+#pragma unroll
+    for (int k = 0; k < NLPL; k++) {
+        const int i = t + k * W;
+        if (i >= n) continue;
+        const double dl = c->dlam[i];
+        if (i >= begin && i < end - 1) {
+            if (sI[i] > -1 && sV[i] > -1) {                       // :330-371
+                double aux = (Ic - ((double)sI[i] + (double)sV[i]));
+                double auxv = (Ic - (double)sI[i]);
+                x += (aux * dl);
+                xv += (auxv * dl);
+                y += aux;
+                yv += auxv;
+                double auxm = (Ic - ((double)sI[i] - (double)sV[i]));
+                xm += (auxm * dl);
+                ym += auxm;
+            }
+            if (sQ[i] > -1 && sU[i] > -1 && sV[i] > -1) {         // :375-393
+                float qu = __fadd_rn(__fmul_rn(sQ[i], sQ[i]), __fmul_rn(sU[i], sU[i]));
+                double Lp = fabs(sqrt((double)qu));
+                double mm = fabs((4 * dl * Lp));
+                double av = fabs((double)sV[i]);
+                xg = xg + av * mm;
+                yg = yg + av * av;
+            }
+        }
+        if (sU[i] > -1 && sQ[i] > -1) {                           // :409-431
+            if (fabs((double)sU[i]) > 0.0001 || fabs((double)sQ[i]) > 0.0001) {
+                su += sU[i];
+                sq += sQ[i];
+            }
+        }
+    }
+    x = group_sum<W>(x); y = group_sum<W>(y); xv = group_sum<W>(xv); yv = group_sum<W>(yv);
+    xm = group_sum<W>(xm); ym = group_sum<W>(ym); xg = group_sum<W>(xg); yg = group_sum<W>(yg);
+    su = group_sum<W>(su); sq = group_sum<W>(sq);
+    const double lplus = (fabs(y) > 1e-15) ? x / y : 0;
+    const double lminus = (fabs(ym) > 1e-15) ? xm / ym : 0;
+    const double C = (PM_CTE4_6_13 * (l0 * l0) * c->geff);
+    const double Blos = (1 / C) * ((lplus - lminus) / 2);
+    double Vlos = (PM_VLIGHT / (l0)) * ((xv / yv) / 2);
+    yg = yg * fabs((3 * C * Blos));
+    const double tan_gamma = fabs(sqrt(xg / yg));
+    double gamma_rad = atan(tan_gamma);
+    double gamma = gamma_rad * (180 / PM_PI);
+    if (init) {
+        if (gamma >= 85 && gamma <= 90) gamma_rad = 85 * (PM_PI / 180);
+        if (gamma > 90 && gamma <= 95) gamma_rad = 95 * (PM_PI / 180);
+    }
+    if (Blos < 0) gamma = (180) - gamma;
+    const double tan2phi = su / sq;
+    double phi = (atan(tan2phi) * 180 / PM_PI) / 2;
+    if (su > 0 && sq > 0) phi = phi;
+    else if (su < 0 && sq > 0) phi = phi + 180;
+    else if (su < 0 && sq < 0) phi = phi + 90;
+    else if (su > 0 && sq < 0) phi = phi + 90;
+    const double B_aux = fabs(Blos / cos(gamma_rad));
+    if (Vlos < (-4)) Vlos = -4;
+    if (Vlos > (4)) Vlos = 4;
+    Classic r;
+    r.B = (B_aux > 4000 ? 4000 : B_aux);
+    r.vlos = Vlos;
+    r.gm = gamma;
+    r.az = phi;
+    r.blos = Blos;
+    if (init) {
+        if (isnan(r.B)) r.B = 1;
+        if (isnan(r.vlos)) r.vlos = 1e-3;
+        if (isnan(r.gm)) r.gm = 1;
+        if (isnan(r.az)) r.az = 1;
+    }
+    return r;
+}
+

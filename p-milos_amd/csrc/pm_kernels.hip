@@ -829,7 +829,7 @@ pm_invert_kernel(InvArgs a)
                     for (int i = 0; i < 11; i++) lm[i] = lm[11 + i] = c->guess[i];
                 wave_sync();
             } else {
-                Classic e = classical(c, sobs, true);
+                Classic e = classical_group<W, NLPL>(c, sobs, true, t);
                 wave_sync();
                 if (t == 0) {
                     for (int i = 0; i < 11; i++) lm[i] = c->guess[i];
