@@ -589,9 +589,11 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
     // form (its register-blocked windows stay small)
     constexpr int R = (!GEN && sizeof(T) == 4) ? 4 : Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
     constexpr int PPC = R / 4;
-    // rows 0..8 but S0's and S1's: S1's comes from the convolved spectrum
-    // (below), or, in the FFT mode without macroturbulence, a pass of its own
-    constexpr int NPS = 7;
+    // rows 0..5 in full passes, the azimuth's Q and U rows in a pass of two
+    // (its I and V rows are zero), S0's never; S1's comes from the convolved
+    // spectrum (below), or, in the FFT mode without macroturbulence, a pass
+    // of its own
+    constexpr int NPS = 6;
     constexpr int NFULL = NPS / PPC;
     const bool mac = GEN && ex.mac;
     if (mac && NT >= 10) {                      // v_mac row from the raw synthesis
@@ -628,6 +630,15 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
 #pragma unroll
                     for (int s = 0; s < 4; s++) D[k][s][p] = x[4 * u + s][k];
             }
+        }
+        {                                       // the azimuth's Q and U rows
+            T x[2][NLPL];
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) { x[0][k] = D[k][1][6]; x[1][k] = D[k][2][6]; }
+            if (mac) convMac<T, W, NLPL, 2>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
+            else convPSF<T, W, NLPL, 2, GEN, false>(c, pad, t, L, x);
+#pragma unroll
+            for (int k = 0; k < NLPL; k++) { D[k][1][6] = x[0][k]; D[k][2][6] = x[1][k]; }
         }
         if (NPS % PPC) {                        // the last parameter alone
             constexpr int q = NFULL * PPC, p = q < 7 ? q : q + 1;
@@ -929,8 +940,7 @@ pm_invert_kernel(InvArgs a)
             bool acc = (c2 - ochisqr < 0.);
             if (acc) {                                         // :581-606
                 flambda = flambda / (5.0 * factor);
-                if (t == 0)
-                    for (int i = 0; i < 11; i++) lm[i] = lm[11 + i];
+                for (int i = t; i < 11; i += W) lm[i] = lm[11 + i];
                 wave_sync();
                 ochisqr = c2;
                 nacc++;
@@ -1042,8 +1052,8 @@ pm_invert_kernel(InvArgs a)
         }
 
         if (finished) {
+            for (int i = t; i < 11; i += W) a.models[(size_t)pix * 11 + i] = lm[i];
             if (t == 0) {
-                for (int i = 0; i < 11; i++) a.models[(size_t)pix * 11 + i] = lm[i];
                 a.chi2[pix] = (float)ochisqr;                   // :625
                 a.iters[pix] = it;
                 if (a.accepted) a.accepted[pix] = nacc;
