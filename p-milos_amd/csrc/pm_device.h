@@ -34,6 +34,12 @@ struct PmConst {
     double lambda0, geff;
     double noise, inv_noise, ilambda, toplim, mu;
     double weight[4];
+    // sqrt(weight[s] / sigma^2) of J^T J (lib.c:87-101): [0] with
+    // 1/sigma^2 = 1/noise, [1] for a missing sample (1/sigma^2 = 1e-21)
+    double swt[2][4];
+    // J^T J entries (packed lower alpha, then beta) of fixed parameters, a
+    // bit each: [0] for the 9-row layout, [1] for the 11-row one
+    unsigned int zbits[2][4];
     double guess[11];
     double nup[PM_MAXC], wep[PM_MAXC];
     double nub[PM_MAXC], web[PM_MAXC], wer[PM_MAXC];

@@ -503,13 +503,11 @@ struct Extra {
     Taps tp;             // tap windows of the macroturbulence kernels
 };
 
-// Zero the response-function row of a fixed parameter,
-// FijaACeroDerivadasNoNecesarias, milosUtils.c:199-227.
-__device__ __forceinline__ bool row_is_fixed(PmC c, int NT, int p)
+// Response rows that are 0 by construction: the azimuth's on I and V
+// (me_der.c:309-312) and S0's off I, never convolved (me_der.c:432).
+__device__ __forceinline__ constexpr bool lit0(int p, int s)
 {
-    if (NT == 9 || NT == 11) return c->fix[p] == 0;
-    if (p < 9) return c->fix[p] == 0;
-    return c->fix[9] == 0 && c->fix[10] == 0;
+    return (p == 6 && (s == 0 || s == 3)) || (p == 7 && s != 0);
 }
 
 // Synthesis of model m for this lane's wavelength slots: fills the sample
@@ -964,21 +962,15 @@ pm_invert_kernel(InvArgs a)
             rf_slots<T, W, NLPL, NT, GEN>(c, lm, slp, g, sp, pad, gm, gd, t, L, ex, first, S, D);
             // covarm2 (lib.c:71-116): each alpha/beta entry is a per-lane
             // partial sum followed by the group sum; lane 0 stores it in LDS
-#pragma unroll
-            for (int k = 0; k < NLPL; k++) {
-                int lam = t + k * W;
-#pragma unroll
-                for (int p = 0; p < NT; p++) {
-                    bool z = lam >= L || row_is_fixed(c, NT, p);
-#pragma unroll
-                    for (int s = 0; s < 4; s++) if (z) D[k][s][p] = T(0);
-                }
-            }
             typedef typename AccT<T>::type A;
             // the weights w/sigma^2 are folded into the rows once, as their
             // square roots, so that every J^T J term is one product; the
-            // gradient takes sqrt(w/sigma^2)(S - obs).  The literal zeros of
-            // the azimuth I/V rows and of the S0 row off I stay literal.
+            // gradient takes sqrt(w/sigma^2)(S - obs).  Slots beyond L get
+            // weight 0 (their rows repeat sample L-1, finite, so every term
+            // is an exact 0).  The literal zeros of the azimuth I/V rows and
+            // of the S0 row off I stay literal, and their products are not
+            // formed (each would add an exact 0).  Rows of fixed parameters
+            // are zeroed after the sums (zbits).
             A rs[NLPL][4];
 #pragma unroll
             for (int k = 0; k < NLPL; k++) {
@@ -986,12 +978,11 @@ pm_invert_kernel(InvArgs a)
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
                     const float o = lam < L ? sobs[s * L + lam] : 0.0f;
-                    double isg = o <= -1.0f ? 1e-21 : c->inv_noise;
-                    const A sw = A(sqrt(c->weight[s] * isg));
+                    const A sw = lam < L ? A(c->swt[o <= -1.0f ? 1 : 0][s]) : A(0);
                     rs[k][s] = sw * A((double)S[s][k] - (double)o);
 #pragma unroll
                     for (int p = 0; p < NT; p++) {
-                        if ((p == 6 && (s == 0 || s == 3)) || (p == 7 && s != 0)) continue;
+                        if (lit0(p, s)) continue;
                         D[k][s][p] = T((A)D[k][s][p] * sw);
                     }
                 }
@@ -1001,6 +992,7 @@ pm_invert_kernel(InvArgs a)
             // share of the lanes (the shares combined by a cross-row swizzle /
             // shuffle) and one lane per entry writes it for the solve.
             constexpr int CH = LD::CH, NV = NP + NT, NCH = (NV + CH - 1) / CH;
+            static_assert((NT == 9 || NT == 11) && 32 % CH == 0, "J^T J layout");
             constexpr int PARTS = W / CH;
 #pragma unroll
             for (int ch = 0; ch < NCH; ch++) {
@@ -1043,6 +1035,9 @@ pm_invert_kernel(InvArgs a)
                 if (PARTS >= 2) sum += swz16_d(sum);
                 if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
                 const int e = ch * CH + vv;
+                // a chunk's bits lie in one word (CH divides 32)
+                const unsigned zw = c->zbits[NT == 11 ? 1 : 0][(ch * CH) >> 5];
+                if ((zw >> (e & 31)) & 1u) sum = 0.0;
                 if (part == 0 && e < NV) ab[e] = sum;
                 wave_sync();
             }
@@ -1324,7 +1319,25 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     c.ilambda = cfg->ilambda;
     c.toplim = cfg->toplim;
     c.mu = cfg->mu;
-    for (int i = 0; i < 4; i++) c.weight[i] = cfg->weight[i];
+    for (int i = 0; i < 4; i++) {
+        c.weight[i] = cfg->weight[i];
+        c.swt[0][i] = sqrt(c.weight[i] * c.inv_noise);
+        c.swt[1][i] = sqrt(c.weight[i] * 1e-21);
+    }
+    // J^T J entries of fixed parameters, zeroed after the sums: the rows of
+    // FijaACeroDerivadasNoNecesarias (milosUtils.c:199-227); in the 9- and
+    // 11-row layouts row p is fixed when fix[p] == 0
+    for (int lay = 0; lay < 2; lay++) {
+        const int nt = lay ? 11 : 9, np = nt * (nt + 1) / 2;
+        for (int i = 0; i < nt; i++) {
+            const bool fi = c.fix[i] == 0;
+            for (int h = 0; h <= i; h++) {
+                const int e = i * (i + 1) / 2 + h;
+                if (fi || c.fix[h] == 0) c.zbits[lay][e >> 5] |= 1u << (e & 31);
+            }
+            if (fi) c.zbits[lay][(np + i) >> 5] |= 1u << ((np + i) & 31);
+        }
+    }
     for (int i = 0; i < L; i++) {
         c.lam[i] = cfg->lambda[i];
         c.dlam[i] = cfg->lambda[i] - cfg->lambda0;
