@@ -64,3 +64,17 @@ def test_error_codes_without_device_calls(lib):
     assert lib.pm_strerror(-22) == b"invalid argument"
     assert lib.pm_grid(FE6173, 0.0, 0.0, 10.0, None, 0) == -22
     assert lib.pm_psf_gaussian(0.0, None, 0, None) == -22
+
+
+def test_jtj_probe_library_rows():
+    """The J^T J probe library loads without a device, and the rows it
+    generates are bounded, zero beyond 30 wavelengths, differ per pixel."""
+    from pmilos import jtj_probe
+    L = jtj_probe._lib()
+    r = np.zeros((10, 4, 32))
+    L.pmp_jtj_rows(5, 3, r.ctypes.data)
+    assert np.all(np.abs(r) <= 0.5) and np.all(r[:, :, 30:] == 0)
+    assert np.count_nonzero(r[:, :, :30]) > 1000
+    r2 = np.zeros_like(r)
+    L.pmp_jtj_rows(6, 3, r2.ctypes.data)
+    assert not np.array_equal(r, r2)
