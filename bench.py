@@ -41,7 +41,7 @@ PEAK_FP32_TFLOPS = 157.3
 # 1048576 pixels (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors
 # a plain 512 MiB streaming kernel reads in the same runs); the summaries
 # are profiles/pm_invert_kernel_l30_1048576px_hbm.md
-TRAFFIC_B_PER_PX = {"fp64": 601.99, "mixed": 620.14}
+TRAFFIC_B_PER_PX = {"fp64": 607.72, "mixed": 660.51}
 ALGO_B_PER_PX = 4 * 30 * 4 + 11 * 8 + 3 * 4     # Stokes in; model, chi2, iters, accepted out
 
 
