@@ -11,7 +11,9 @@ as float32.  Inversion: 9 free parameters, <= 50 LM iterations, the run/
 guess.mod initial model plus classical estimates per pixel.
 
 One step = one inversion pass over the whole cube, inputs resident in HBM.
-After the FP64 and mixed passes, a bounded streaming run (pmilos.serving:
+The same field at noise sigma 1e-4 and 3e-3 Ic gives one more pass of each
+form (the mixed form's error against the FP64 form and the change in
+iteration count per sigma).  After these, a bounded streaming run (pmilos.serving:
 Poisson arrivals of scans of 32^2..2048^2 pixels at 6..112 wavelengths
 through the serving loop, mixed form) gives p50/p99 scan latency.
 
@@ -169,6 +171,43 @@ def uneven_batch(args, torch, pmilos, precision):
             "mean_iters": [round(x, 3) for x in iters]}
 
 
+# the mixed form's stated per-parameter tolerance against the FP64 form
+# (absolute): eta0, B [G], vlos [km/s], dopp [A], a, gamma [deg], phi [deg],
+# S0, S1; the GPU tests hold the mixed form to it against the FP64 oracle
+TOL_MIXED = np.array([0.5, 2.0, 2e-3, 2e-4, 5e-3, 0.2, 2.0, 1e-2, 1e-2])
+
+
+def noise_sweep(done, make_cube, run_both):
+    """The field at noise sigma 1e-4, 1e-3 and 3e-3 Ic (noise seeds 31, 32,
+    33; the models of the main field): per sigma one FP64 and one mixed pass
+    with the kernel shapes already chosen, the mixed form's per-parameter
+    error against the FP64 form and the change in iteration count."""
+    names = ["eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1"]
+    out = {}
+    for sigma, seed in ((1e-4, 31), (1e-3, 32), (3e-3, 33)):
+        if sigma in done:
+            r64, rmx = done[sigma]
+        else:
+            cube = make_cube(sigma, seed)
+            r64, rmx = run_both(cube)
+            del cube
+        d = np.abs(rmx["models"][:, :9] - r64["models"][:, :9])
+        di = rmx["iters"] - r64["iters"]
+        out["%g" % sigma] = {
+            "fp64_pixels_per_s": round(len(d) / (r64["kernel_ms"] * 1e-3), 1),
+            "mixed_pixels_per_s": round(len(d) / (rmx["kernel_ms"] * 1e-3), 1),
+            "median_abs_diff": dict(zip(names, [float("%.3g" % v) for v in np.median(d, 0)])),
+            "p99_abs_diff": dict(zip(names, [float("%.3g" % v) for v in np.percentile(d, 99, 0)])),
+            "frac_within_tol": dict(zip(names, [round(float(v), 5)
+                                                for v in (d <= TOL_MIXED).mean(0)])),
+            "mean_iters_fp64": round(float(r64["iters"].mean()), 3),
+            "mean_iters_mixed": round(float(rmx["iters"].mean()), 3),
+            "frac_iters_identical": round(float((di == 0).mean()), 4),
+            "mean_abs_iter_change": round(float(np.abs(di).mean()), 3),
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +225,8 @@ def main():
     ap.add_argument("--serve-util", type=float, default=0.5)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default kernel shape instead of the autotuner's")
+    ap.add_argument("--no-sweep", dest="sweep", action="store_false",
+                    help="skip the noise sweep (sigma 1e-4 and 3e-3 Ic) of the two forms")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
                     help="skip the uneven-batch (one graph launch) measurement")
     args = ap.parse_args()
@@ -212,14 +253,17 @@ def main():
 
     # synthetic cube: library FP64 synthesis, float64 noise, stored float32
     models = torch.from_numpy(draw_models(npix, 2)).to(dev)
-    spec = torch.empty((npix, 4, L), dtype=torch.float64, device=dev)
-    ctx.synth_ptrs(models.data_ptr(), npix, spec.data_ptr(), 0, sptr)
-    gen = torch.Generator(device=dev)
-    gen.manual_seed(32)
-    ic = (models[:, 7] + models[:, 8]).view(-1, 1, 1)
-    noise = torch.randn((npix, 4, L), dtype=torch.float64, device=dev, generator=gen)
-    stokes = (spec + noise * 1e-3 * ic).to(torch.float32).contiguous()
-    del spec, noise
+
+    def make_cube(sigma, seed):
+        spec = torch.empty((npix, 4, L), dtype=torch.float64, device=dev)
+        ctx.synth_ptrs(models.data_ptr(), npix, spec.data_ptr(), 0, sptr)
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(seed)
+        ic = (models[:, 7] + models[:, 8]).view(-1, 1, 1)
+        noise = torch.randn((npix, 4, L), dtype=torch.float64, device=dev, generator=gen)
+        return (spec + noise * sigma * ic).to(torch.float32).contiguous()
+
+    stokes = make_cube(1e-3, 32)
     out_m = torch.empty((npix, 11), dtype=torch.float64, device=dev)
     out_c = torch.empty(npix, dtype=torch.float32, device=dev)
     out_i = torch.empty(npix, dtype=torch.int32, device=dev)
@@ -234,36 +278,39 @@ def main():
         del src, dst
     torch.cuda.synchronize()
 
-    def run_form(c, peak):
+    def run_form(c, peak, cube=None, steps=None, warmup=None, tune=True):
+        cube = stokes if cube is None else cube
+        steps = args.steps if steps is None else steps
+        warmup = args.warmup if warmup is None else warmup
         # kernel shape by the autotuner, on the first 65536 pixels (untimed)
-        shape = c.autotune_ptrs(stokes.data_ptr(), min(npix, 65536), sptr) if args.autotune \
-            else c.variant()[0]
+        shape = c.autotune_ptrs(cube.data_ptr(), min(npix, 65536), sptr) \
+            if args.autotune and tune else c.variant()[0]
 
         def step():
-            c.invert_ptrs(stokes.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
+            c.invert_ptrs(cube.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
                           out_i.data_ptr(), out_a.data_ptr(), 0, sptr)
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             step()
         torch.cuda.synchronize()
         ev0 = torch.cuda.Event(enable_timing=True)
         ev1 = torch.cuda.Event(enable_timing=True)
         t0 = time.perf_counter()
         ev0.record(stream)
-        for _ in range(args.steps):
+        for _ in range(steps):
             step()
         ev1.record(stream)
         torch.cuda.synchronize()
         wall = time.perf_counter() - t0
-        kern_ms = ev0.elapsed_time(ev1) / args.steps
+        kern_ms = ev0.elapsed_time(ev1) / steps
         iters = out_i.cpu().numpy().astype(np.float64)
         acc = out_a.cpu().numpy().astype(np.float64)
         fl = float(np.sum(flops.pixel_flops(iters, acc, L=L, NT=9)))
         achieved = fl / (kern_ms * 1e-3) / 1e12
-        return dict(value=npix * args.steps / wall, ms_per_step=wall * 1000.0 / args.steps,
+        return dict(value=npix * steps / wall, ms_per_step=wall * 1000.0 / steps,
                     shape=list(shape),
                     kernel_ms=kern_ms, achieved=achieved, frac=achieved / peak,
                     mean_iters=float(iters.mean()), mean_accepted=float(acc.mean()),
-                    fallbacks=c.fallback_count(), models=out_m.cpu().numpy())
+                    fallbacks=c.fallback_count(), models=out_m.cpu().numpy(), iters=iters)
 
     if args.form == "mixed":
         ctx = pmilos.Context(FE6173, TRIPLET, lam, psf=G, guess=GUESS, noise=1e-6, miter=50,
@@ -313,6 +360,11 @@ def main():
             "p99_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
                                              [float("%.3g" % v) for v in np.percentile(d, 99, 0)])),
         }
+        if args.sweep:
+            line["mixed"]["noise_sweep"] = noise_sweep(
+                {1e-3: (r64, rmx)}, lambda sg, sd: make_cube(sg, sd),
+                lambda cube: (run_form(ctx, PEAK_FP64_TFLOPS, cube, 1, 0, False),
+                              run_form(cm, PEAK_FP32_TFLOPS, cube, 1, 0, False)))
     if args.batch:
         line["uneven_batch"] = {"fp64": uneven_batch(args, torch, pmilos, pmilos.PM_FP64),
                                 "mixed": uneven_batch(args, torch, pmilos, pmilos.PM_MIXED)}

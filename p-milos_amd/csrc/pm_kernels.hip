@@ -587,10 +587,9 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
     // form (its register-blocked windows stay small)
     constexpr int R = (!GEN && sizeof(T) == 4) ? 4 : Lds<T, W, NLPL, NT, GEN>::PAD_ROWS;
     constexpr int PPC = R / 4;
-    // rows 0..5 in full passes, the azimuth's Q and U rows in a pass of two
-    // (its I and V rows are zero), S0's never; S1's comes from the convolved
-    // spectrum (below), or, in the FFT mode without macroturbulence, a pass
-    // of its own
+    // rows 0..5 in full passes, S0's never; the azimuth's and S1's come
+    // from the convolved spectrum (below), S1's in the FFT mode without
+    // macroturbulence from a pass of its own
     constexpr int NPS = 6;
     constexpr int NFULL = NPS / PPC;
     const bool mac = GEN && ex.mac;
@@ -628,15 +627,6 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
 #pragma unroll
                     for (int s = 0; s < 4; s++) D[k][s][p] = x[4 * u + s][k];
             }
-        }
-        {                                       // the azimuth's Q and U rows
-            T x[2][NLPL];
-#pragma unroll
-            for (int k = 0; k < NLPL; k++) { x[0][k] = D[k][1][6]; x[1][k] = D[k][2][6]; }
-            if (mac) convMac<T, W, NLPL, 2>(pad, gm, ex.tp.mlo, ex.tp.mhi, t, L, x);
-            else convPSF<T, W, NLPL, 2, GEN, false>(c, pad, t, L, x);
-#pragma unroll
-            for (int k = 0; k < NLPL; k++) { D[k][1][6] = x[0][k]; D[k][2][6] = x[1][k]; }
         }
         if (NPS % PPC) {                        // the last parameter alone
             constexpr int q = NFULL * PPC, p = q < 7 ? q : q + 1;
@@ -679,6 +669,24 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
                     const T sc = (GEN && slp) ? ex.Ssl[s][k] : S[s][k];
                     D[k][s][8] = T(((A)sc - (s == 0 ? s0 : A(0))) * is1);
                 }
+        }
+    }
+    {
+        // The azimuth turns the frame of the linear polarisation: Q + iU of
+        // the model at azimuth phi is that at 0 times exp(2i phi), the same
+        // at every wavelength, so d(Q, U)/d(phi) = (-2U, 2Q) per radian.  The
+        // rows of me_der.c:183-199 and :205-277 (through the convolutions,
+        // which are linear and act per row) are therefore (-2U, 2Q) pi/180
+        // of this model's convolved spectrum before any stray-light mix; the
+        // mix then scales them by alpha below like every other row.
+        typedef typename AccT<T>::type A;
+        const A tw = A(2.0 * PM_PI / 180.0);
+#pragma unroll
+        for (int k = 0; k < NLPL; k++) {
+            const T q = (GEN && slp) ? ex.Ssl[1][k] : S[1][k];
+            const T u = (GEN && slp) ? ex.Ssl[2][k] : S[2][k];
+            D[k][1][6] = T(-tw * (A)u);
+            D[k][2][6] = T(tw * (A)q);
         }
     }
     if (GEN && slp) {

@@ -123,9 +123,11 @@ def test_cli_per_inversion(work, fft):
     assert np.all(np.abs(m_cli[:9] - o["model"][:9]) <= np.maximum(TOL64, 1e-6))
     np.testing.assert_allclose(chi_cli, o["chi2"], rtol=1e-5)
     # the GPU's sums and solve round differently from the oracle's, and the
-    # "chi2 unchanged" exit (|dchi2|/chi2 < 1e-12) may come a few rejected
-    # trips apart once the model has converged
-    assert abs(it_cli - o["iters"]) <= 3
+    # "chi2 unchanged" exit (|dchi2| * 100 / chi2 < toplim = 1e-12, i.e.
+    # about 100 ulps of chi2) may come several trips apart once the model
+    # has converged: steps at the rounding level decide it, and each
+    # rejection raises the damping tenfold (measured: up to 4 trips here)
+    assert abs(it_cli - o["iters"]) <= 6
     np.testing.assert_allclose(fit, o["fit"], rtol=1e-4, atol=1e-6)
 
 

@@ -237,6 +237,51 @@ def test_response_functions_mac_and_filling_factor(psf):
         np.testing.assert_allclose(rf[:, 10], sp_pure - slight, rtol=1e-12, atol=1e-14)
 
 
+@pytest.mark.parametrize("L0,g", [(FE6173, (-290, 20, 290)), (FE6173, (-300, 20, 300)),
+                                  (6302.4936, (-1193.25, 21.5, 1193.25))])
+def test_azimuth_rows_are_the_rotation_of_the_spectrum(L0, g):
+    """The azimuth only turns the frame of the linear polarisation, Q + iU ->
+    (Q + iU) exp(2i phi), so its response rows are d(Q, U)/d(phi) =
+    (-2U, 2Q) pi/180 of the same model's spectrum (before any stray-light
+    mix, times alpha after it), with or without PSF, FFT mode and
+    macroturbulence: the convolutions are linear and act per row.  The
+    kernels form the rows this way; a sign, a missing factor 2 or a
+    degree/radian slip in either the oracle's rows or this identity fails."""
+    lam = O.grid(L0, *g)
+    L = len(lam)
+    G = O.gauss_psf(60.0, lam)
+    rng = np.random.default_rng(11)
+    slight = np.abs(rng.normal(0.5, 0.2, (4, L)))
+    slight[1:] *= 0.01
+    f = np.pi / 180
+    cases = [(dict(psf=None), False, False), (dict(psf=G), False, False),
+             (dict(psf=G, use_fft=True), False, False),
+             (dict(psf=G, fix=[1] * 10 + [0], guess_mac=0.5), True, False),
+             (dict(psf=G, fix=[1] * 10 + [0], guess_mac=0.5, use_fft=True), True, False),
+             (dict(psf=G, fix=[1] * 9 + [0, 1], slight=slight), False, True),
+             (dict(psf=G, fix=[1] * 11, guess_mac=0.5, slight=slight), True, True)]
+    for kw, mac, sl in cases:
+        s = O.make_setup(L0, TRIPLET, lam, **kw)
+        for m in draw_models(12, seed=5):
+            m = m.copy()
+            if mac:
+                m[9] = rng.uniform(0.2, 2.0)
+            if sl:
+                m[10] = rng.uniform(0.5, 0.95)
+                # the pre-mix spectrum: the oracle without the stray light
+                s_pre = O.make_setup(L0, TRIPLET, lam, **{k: v for k, v in kw.items()
+                                                          if k != "slight"})
+                pre = O.synth(s_pre, m)
+            sp, d = O.synth(s, m, rf=True)
+            alf = m[10] if sl else 1.0
+            if not sl:
+                pre = sp
+            sc = max(np.abs(d[1:3, 6]).max(), 1e-30)
+            assert np.abs(d[1, 6] - alf * (-2 * f) * pre[2]).max() <= 1e-13 * sc, (kw, m)
+            assert np.abs(d[2, 6] - alf * (2 * f) * pre[1]).max() <= 1e-13 * sc, (kw, m)
+            assert np.all(d[0, 6] == 0) and np.all(d[3, 6] == 0)
+
+
 def test_response_functions_psf_is_row_convolution():
     """With a PSF, QUV rows are the zero-padded correlation of the raw rows and
     I rows use the Ic trick with each row's own max(first, last); the S0 row is
