@@ -43,7 +43,12 @@ PEAK_FP32_TFLOPS = 157.3
 # 1048576 pixels (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors
 # a plain 512 MiB streaming kernel reads in the same runs); the summaries
 # are profiles/pm_invert_kernel_l30_1048576px_hbm.md
-TRAFFIC_B_PER_PX = {"fp64": 607.72, "mixed": 660.51}
+TRAFFIC_B_PER_PX = {"fp64": 605.29, "mixed": 635.72}
+# lane FLOPs issued per pixel by the same launches (64 x the
+# SQ_INSTS_VALU_FLOPS_FP64 / _FP32 counters: every lane slot of an issued
+# instruction); the summary is profiles/pm_invert_kernel_l30_1048576px_valu.md
+ISSUED_FLOP_PER_PX = {"fp64": {"fp64": 4808470, "fp32": 189},
+                      "mixed": {"fp64": 759735, "fp32": 3260667}}
 ALGO_B_PER_PX = 4 * 30 * 4 + 11 * 8 + 3 * 4     # Stokes in; model, chi2, iters, accepted out
 
 
@@ -312,6 +317,12 @@ def main():
                     mean_iters=float(iters.mean()), mean_accepted=float(acc.mean()),
                     fallbacks=c.fallback_count(), models=out_m.cpu().numpy(), iters=iters)
 
+    def issued(r, form):
+        # counter-measured issue rate of this pass (the per-pixel figures are
+        # those of the bench field; other --npix values scale them)
+        return {k: v * npix / (r["kernel_ms"] * 1e-3) / 1e12
+                for k, v in ISSUED_FLOP_PER_PX[form].items()}
+
     if args.form == "mixed":
         ctx = pmilos.Context(FE6173, TRIPLET, lam, psf=G, guess=GUESS, noise=1e-6, miter=50,
                              precision=pmilos.PM_MIXED)
@@ -330,9 +341,13 @@ def main():
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(r64["frac"], 4),
                      "traffic": round(TRAFFIC_B_PER_PX["fp64"] * npix),
+                     "issued_fp64_tflops": round(issued(r64, "fp64")["fp64"], 3),
+                     "issued_frac": round(issued(r64, "fp64")["fp64"] / PEAK_FP64_TFLOPS, 4),
                      "note": "FP64 vector ALU work (the MI355X FP64 vector and matrix "
                              "peaks are both 78.6 TFLOP/s); achieved = algorithmic "
-                             "FLOPs of the pass / pm_invert_kernel time; traffic = HBM "
+                             "FLOPs of the pass / pm_invert_kernel time; issued = lane "
+                             "FLOPs issued per pixel from the SQ_INSTS_VALU_FLOPS "
+                             "counters / the same time; traffic = HBM "
                              "bytes per launch from the FETCH_SIZE/WRITE_SIZE counters, "
                              "%.0f B/pixel against %d algorithmic" % (TRAFFIC_B_PER_PX["fp64"],
                                                                      ALGO_B_PER_PX)},
@@ -353,6 +368,8 @@ def main():
             "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup": rmx["shape"],
             "roofline": {"achieved": round(rmx["achieved"], 3), "peak": PEAK_FP32_TFLOPS,
                          "unit": "TFLOP/s", "frac": round(rmx["frac"], 4),
+                         "issued_fp32_tflops": round(issued(rmx, "mixed")["fp32"], 3),
+                         "issued_fp64_tflops": round(issued(rmx, "mixed")["fp64"], 3),
                          "traffic": round(TRAFFIC_B_PER_PX["mixed"] * npix)},
             "mean_iters": round(rmx["mean_iters"], 3),
             "median_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
