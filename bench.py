@@ -11,6 +11,9 @@ as float32.  Inversion: 9 free parameters, <= 50 LM iterations, the run/
 guess.mod initial model plus classical estimates per pixel.
 
 One step = one inversion pass over the whole cube, inputs resident in HBM.
+Beside it: one pass of each form from the cube in pinned host memory to the
+results in host memory (PCIe copies included), and one pass of each form
+with the logclambda damping rule of run/pmilos.mtrol (logclambda = 1).
 The same field at noise sigma 1e-4 and 3e-3 Ic gives one more pass of each
 form (the mixed form's error against the FP64 form and the change in
 iteration count per sigma).  After these, a bounded streaming run (pmilos.serving:
@@ -230,6 +233,8 @@ def main():
     ap.add_argument("--serve-util", type=float, default=0.5)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default kernel shape instead of the autotuner's")
+    ap.add_argument("--no-extra", dest="extra", action="store_false",
+                    help="skip the host-to-host (pinned) pass and the logclambda=1 passes")
     ap.add_argument("--no-sweep", dest="sweep", action="store_false",
                     help="skip the noise sweep (sigma 1e-4 and 3e-3 Ic) of the two forms")
     ap.add_argument("--no-batch", dest="batch", action="store_false",
@@ -317,6 +322,41 @@ def main():
                     mean_iters=float(iters.mean()), mean_accepted=float(acc.mean()),
                     fallbacks=c.fallback_count(), models=out_m.cpu().numpy(), iters=iters)
 
+    def host_to_host(c):
+        """One pass from the float32 cube in pinned host memory to the results
+        in pinned host memory (upload, inversion, copy-back on one stream),
+        wall clock; and the copies alone."""
+        h_in = torch.empty(stokes.shape, dtype=torch.float32, pin_memory=True)
+        h_in.copy_(stokes)
+        d_in = torch.empty_like(stokes)
+        outs = (out_m, out_c, out_i, out_a)
+        h_out = [torch.empty(o.shape, dtype=o.dtype, pin_memory=True) for o in outs]
+
+        def copies():
+            d_in.copy_(h_in, non_blocking=True)
+            for h, o in zip(h_out, outs):
+                h.copy_(o, non_blocking=True)
+
+        def whole():
+            d_in.copy_(h_in, non_blocking=True)
+            c.invert_ptrs(d_in.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
+                          out_i.data_ptr(), out_a.data_ptr(), 0, sptr)
+            for h, o in zip(h_out, outs):
+                h.copy_(o, non_blocking=True)
+
+        res = {}
+        for name, fn in (("copies_only", copies), ("whole", whole)):
+            fn()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            res[name] = time.perf_counter() - t
+        del d_in, h_in, h_out
+        return {"pixels_per_s": round(npix / res["whole"], 1),
+                "ms": round(res["whole"] * 1e3, 3),
+                "copies_only_ms": round(res["copies_only"] * 1e3, 3)}
+
     def issued(r, form):
         # counter-measured issue rate of this pass (the per-pixel figures are
         # those of the bench field; other --npix values scale them)
@@ -376,7 +416,48 @@ def main():
                                                 [float("%.3g" % v) for v in np.median(d, 0)])),
             "p99_abs_diff_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
                                              [float("%.3g" % v) for v in np.percentile(d, 99, 0)])),
+            "frac_within_tol_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
+                                                [round(float(v), 5)
+                                                 for v in (d <= TOL_MIXED).mean(0)])),
+            "toplim_mixed": pmilos.PM_MIXED_TOPLIM,
         }
+        # the same pass with the reference's chi2-unchanged test (toplim alone)
+        cm.set_mixed_toplim(0.0)
+        r0 = run_form(cm, PEAK_FP32_TFLOPS, steps=1, warmup=1, tune=False)
+        cm.set_mixed_toplim(pmilos.PM_MIXED_TOPLIM)
+        d0 = np.abs(r0["models"][:, :9] - r64["models"][:, :9])
+        line["mixed"]["reference_exit_test"] = {
+            "pixels_per_s": round(npix / (r0["kernel_ms"] * 1e-3), 1),
+            "kernel_ms": round(r0["kernel_ms"], 3),
+            "speedup_vs_fp64": round(r64["kernel_ms"] / r0["kernel_ms"], 3),
+            "mean_iters": round(r0["mean_iters"], 3),
+            "frac_within_tol_vs_fp64": dict(zip(pmilos.PARAM_NAMES[:9],
+                                                [round(float(v), 5)
+                                                 for v in (d0 <= TOL_MIXED).mean(0)]))}
+        if args.extra:
+            line["host_to_host"] = {"fp64": host_to_host(ctx), "mixed": host_to_host(cm)}
+            line["logclambda_1"] = {}
+            for name, prec, peak in (("fp64", pmilos.PM_FP64, PEAK_FP64_TFLOPS),
+                                     ("mixed", pmilos.PM_MIXED, PEAK_FP32_TFLOPS)):
+                cl = pmilos.Context(FE6173, TRIPLET, lam, psf=G, guess=GUESS, noise=1e-6,
+                                    miter=50, logclambda=1, precision=prec)
+                r = run_form(cl, peak, steps=1, warmup=1)
+                line["logclambda_1"][name] = {
+                    "pixels_per_s": round(npix / (r["kernel_ms"] * 1e-3), 1),
+                    "kernel_ms": round(r["kernel_ms"], 3),
+                    "mean_iters": round(r["mean_iters"], 3),
+                    "mean_accepted": round(r["mean_accepted"], 3)}
+                cl.close()
+                if name == "fp64":
+                    m64 = r["models"]
+                else:
+                    d = np.abs(r["models"][:, :9] - m64[:, :9])
+                    line["logclambda_1"]["mixed"]["frac_within_tol_vs_fp64"] = dict(
+                        zip(pmilos.PARAM_NAMES[:9],
+                            [round(float(v), 5) for v in (d <= TOL_MIXED).mean(0)]))
+                    line["logclambda_1"]["mixed"]["p99_abs_diff_vs_fp64"] = dict(
+                        zip(pmilos.PARAM_NAMES[:9],
+                            [float("%.3g" % v) for v in np.percentile(d, 99, 0)]))
         if args.sweep:
             line["mixed"]["noise_sweep"] = noise_sweep(
                 {1e-3: (r64, rmx)}, lambda sg, sd: make_cube(sg, sd),

@@ -181,6 +181,20 @@ int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3]);
 /* Selects variant index (0 = the default mapping) of that form by hand. */
 int pm_ctx_set_variant(pm_ctx *ctx, int general, int index);
 
+/* No counterpart in the reference: the mixed form's "chi2 unchanged" exit
+ * test (milosUtils.c:579-580, |ochi2 - chi2| * 100 / chi2 < toplim) uses
+ * max(toplim, toplim_mixed).  A chi2 summed from FP32 spectra cannot
+ * resolve the reference's 1e-14 relative change (the FP32 and FP64 forms'
+ * chi2 of one model differ by up to ~1e-4 relative), so near the minimum
+ * the reference's test only ends the loop once the damping has made the
+ * step smaller than the FP32 rounding of every sample.  Mixed contexts start
+ * with PM_MIXED_TOPLIM (|dchi2| / chi2 < 1e-6); 0 restores the reference's
+ * test.  The FP64 form ignores it.  Call it while no inversion of this
+ * context is running (it rewrites the context's device constants).
+ * 0 <= toplim_mixed < 100, else PM_EINVAL. */
+#define PM_MIXED_TOPLIM 1e-4
+int pm_ctx_set_mixed_toplim(pm_ctx *ctx, double toplim_mixed);
+
 /* One launch for several inversions (e.g. cubes of different samplings):
  * each item is a pm_invert_sl_device call on device buffers.  They are
  * captured into one HIP graph, one concurrent branch per item with its own

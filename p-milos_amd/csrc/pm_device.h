@@ -33,6 +33,10 @@ struct PmConst {
     int fix[11];
     double lambda0, geff;
     double noise, inv_noise, ilambda, toplim, mu;
+    // the mixed form's "chi2 unchanged" threshold (the reference's units,
+    // |dchi2| * 100 / chi2): the test uses the larger of it and toplim; 0
+    // keeps the reference's test
+    double toplim_mixed;
     double weight[4];
     // sqrt(weight[s] / sigma^2) of J^T J (lib.c:87-101): [0] with
     // 1/sigma^2 = 1/noise, [1] for a missing sample (1/sigma^2 = 1e-21)

@@ -942,7 +942,10 @@ pm_invert_kernel(InvArgs a)
             do_rf = true;
             iterating = true;
         } else {
-            bool clanda = (fabs((ochisqr - c2) * 100 / c2) < c->toplim) || (c2 < 0.0001);
+            // the mixed form may test against the resolution of its chi2
+            // (spectra in FP32) instead of the reference's toplim alone
+            const double tl = sizeof(T) == 4 ? fmax(c->toplim, c->toplim_mixed) : c->toplim;
+            bool clanda = (fabs((ochisqr - c2) * 100 / c2) < tl) || (c2 < 0.0001);
             bool acc = (c2 - ochisqr < 0.);
             if (acc) {                                         // :581-606
                 flambda = flambda / (5.0 * factor);
@@ -1326,6 +1329,7 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     c.inv_noise = 1.0 / cfg->noise;
     c.ilambda = cfg->ilambda;
     c.toplim = cfg->toplim;
+    c.toplim_mixed = cfg->precision == PM_MIXED ? PM_MIXED_TOPLIM : 0.0;
     c.mu = cfg->mu;
     for (int i = 0; i < 4; i++) {
         c.weight[i] = cfg->weight[i];
@@ -1682,6 +1686,20 @@ extern "C" int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3])
     out[1] = v[k].NLPL;
     out[2] = v[k].BLK / v[k].W;
     return nv;
+}
+
+extern "C" int pm_ctx_set_mixed_toplim(pm_ctx *ctx, double toplim_mixed)
+{
+    if (!ctx || !(toplim_mixed >= 0.0) || !(toplim_mixed < 100.0)) return PM_EINVAL;
+    ctx->h.toplim_mixed = toplim_mixed;
+    int dev = 0;
+    PM_HIP(hipGetDevice(&dev));
+    PM_HIP(hipSetDevice(ctx->device));
+    const hipError_t e = hipMemcpy(&ctx->d->toplim_mixed, &ctx->h.toplim_mixed, sizeof(double),
+                                   hipMemcpyHostToDevice);
+    PM_HIP(hipSetDevice(dev));
+    PM_HIP(e);
+    return PM_OK;
 }
 
 extern "C" int pm_ctx_set_variant(pm_ctx *ctx, int general, int index)

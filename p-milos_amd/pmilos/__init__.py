@@ -17,6 +17,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libpmilos_hip.so")
 NPAR = 11
 PM_FP64, PM_MIXED = 0, 1
+# the mixed form's default chi2-unchanged threshold (PM_MIXED_TOPLIM)
+PM_MIXED_TOPLIM = 1e-4
 PARAM_NAMES = ["eta0", "B", "vlos", "dopp", "aa", "gm", "az", "S0", "S1", "mac", "alfa"]
 
 _lib = None
@@ -77,6 +79,7 @@ def lib():
         L.pm_ctx_autotune.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_int * 3)]
         L.pm_ctx_variant.argtypes = [vp, C.c_int, C.POINTER(C.c_int * 3)]
         L.pm_ctx_set_variant.argtypes = [vp, C.c_int, C.c_int]
+        L.pm_ctx_set_mixed_toplim.argtypes = [vp, C.c_double]
         L.pm_graph_create.argtypes = [C.POINTER(BatchItem), C.c_int, C.c_double, C.POINTER(vp)]
         L.pm_graph_launch.argtypes = [vp, vp]
         L.pm_graph_destroy.argtypes = [vp]
@@ -233,6 +236,12 @@ class Context:
 
     def set_variant(self, index, general=False):
         _check(lib().pm_ctx_set_variant(self.h, int(general), index))
+
+    def set_mixed_toplim(self, toplim_mixed):
+        """The mixed form's chi2-unchanged threshold (reference units:
+        |dchi2| * 100 / chi2); mixed contexts start at PM_MIXED_TOPLIM, 0
+        restores the reference's toplim test."""
+        _check(lib().pm_ctx_set_mixed_toplim(self.h, C.c_double(toplim_mixed)))
 
     def fallback_count(self):
         v = C.c_longlong(0)

@@ -15,6 +15,7 @@ scans of that kind.
 """
 import argparse
 import json
+import sys
 import threading
 import time
 
@@ -92,10 +93,10 @@ def calibrate(ctxs, pool):
     return rate
 
 
-def run(server, ctxs, pool, kinds, rate_hz, seed=50):
+def run(server, ctxs, pool, kinds, rate_hz, seed=50, progress=False):
     """Submits len(kinds) scans at Poisson arrival times; returns per-scan
     (kind, npix, latency) and the wall time from first arrival to last
-    completion."""
+    completion.  progress: a line on stderr every 50 completed scans."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(kinds))
     tickets = []
@@ -123,6 +124,9 @@ def run(server, ctxs, pool, kinds, rate_hz, seed=50):
             _, lat = server.wait(t)
             res.append((k, k[0] * k[0], lat))
             i += 1
+            if progress and i % 50 == 0:
+                print("served %d/%d scans, %.1f s" % (i, len(kinds), time.perf_counter() - t0),
+                      file=sys.stderr, flush=True)
         elif done.is_set() and i >= len(tickets):
             break
         else:
@@ -146,7 +150,7 @@ def summarize(res, wall):
 
 
 def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SIZES,
-                samplings=tuple(SAMPLINGS)):
+                samplings=tuple(SAMPLINGS), progress=False):
     ctxs = contexts(precision)
     kinds_all = [(s, L) for s in sizes for L in samplings]
     pool = {}
@@ -160,7 +164,7 @@ def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SI
     srv = pmilos.Server()
     try:
         srv.wait(srv.submit(ctxs[30], pool[(32, 30)]))   # first use of its streams
-        res, wall = run(srv, ctxs, pool, kinds, arrival, seed)
+        res, wall = run(srv, ctxs, pool, kinds, arrival, seed, progress)
     finally:
         srv.close()
     out = summarize(res, wall)
@@ -182,7 +186,7 @@ def main():
     torch.cuda.set_device(0)
     prec = pmilos.PM_MIXED if args.precision == "mixed" else pmilos.PM_FP64
     for u in args.util:
-        print(json.dumps(serve_bench(prec, args.scans, u, args.seed)), flush=True)
+        print(json.dumps(serve_bench(prec, args.scans, u, args.seed, progress=True)), flush=True)
 
 
 if __name__ == "__main__":

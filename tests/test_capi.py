@@ -64,6 +64,8 @@ def test_error_codes_without_device_calls(lib):
     assert lib.pm_strerror(-22) == b"invalid argument"
     assert lib.pm_grid(FE6173, 0.0, 0.0, 10.0, None, 0) == -22
     assert lib.pm_psf_gaussian(0.0, None, 0, None) == -22
+    lib.pm_ctx_set_mixed_toplim.argtypes = [ctypes.c_void_p, ctypes.c_double]
+    assert lib.pm_ctx_set_mixed_toplim(None, 1e-3) == -22
 
 
 def test_jtj_probe_library_rows():

@@ -63,3 +63,31 @@ def test_mixed_inversion_vs_fp64_oracle(sigma, seed):
     say("sigma %g: iterations mixed %.2f fp64 %.2f, identical %.3f"
         % (sigma, g["iters"].mean(), o["iters"].mean(), (g["iters"] == o["iters"]).mean()))
     assert np.all(frac >= 0.99), frac
+
+
+def test_mixed_exit_threshold():
+    """Mixed contexts start with PM_MIXED_TOPLIM as their chi2-unchanged
+    threshold; 0 restores the reference's toplim test.  Both stay within
+    TOLMIX of the FP64 oracle, the default ends no later on average, and the
+    FP64 form ignores the setting (same bits)."""
+    say("case mixed exit threshold")
+    s, ctx = _setup()
+    truths = draw_models(1024, seed=2)
+    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=32)
+    o = O.invert(s, cube, GUESS)
+    g_def = ctx.invert(cube)
+    ctx.set_mixed_toplim(0.0)
+    g_ref = ctx.invert(cube)
+    for name, g in (("default", g_def), ("reference test", g_ref)):
+        frac = (np.abs(g["model"][:, :9] - o["model"][:, :9]) <= TOLMIX).mean(axis=0)
+        say("%s: fraction within tolerance %s, mean iters %.2f (oracle %.2f)"
+            % (name, np.round(frac, 4), g["iters"].mean(), o["iters"].mean()))
+        assert np.all(frac >= 0.99), (name, frac)
+    assert g_def["iters"].mean() <= g_ref["iters"].mean()
+    lam = O.grid(FE6173, -290, 20, 290)
+    c64 = pmilos.Context(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam), guess=GUESS)
+    a = c64.invert(cube[:256])
+    c64.set_mixed_toplim(1e-2)
+    b = c64.invert(cube[:256])
+    for k in ("model", "iters", "chi2"):
+        assert np.array_equal(a[k], b[k]), k
