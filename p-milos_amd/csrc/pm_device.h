@@ -18,7 +18,7 @@
 
 #define PM_MAXC 16          // Zeeman components per group (pi / blue / red)
 #define PM_LMAX 128
-#define PM_HC 12            // PSF half-width of the register-blocked correlation
+#define PM_HC 11            // PSF half-width of the register-blocked correlation
 #define PM_PI 3.14159265358979323846264338327950288419716939937510
 #define PM_RR 0.5641895836          // defines.h:73 (truncated 1/sqrt(pi))
 #define PM_VLIGHT 2.99792458e+5     // defines.h:75

@@ -133,10 +133,12 @@ __device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
 
 // Half-width of the register-blocked correlation for rows of NX samples
 // (the window start half - HC stays inside the row buffer for L/2 >= HC):
-// 12 taps each side in the FP64 form, 9 in the FP32 form, whose tap range
-// ends at 1e-9 of the peak (a 60 mA Gaussian on a 20 mA grid needs 11 and 8)
+// 11 taps each side in the FP64 form (taps down to 1e-17 of the peak), 8 in
+// the FP32 form (down to 1e-9).  That covers Gaussian PSFs narrower than
+// 3.19 (FP64) and 3.29 (FP32) samples FWHM; a 60 mA Gaussian on a 20 mA grid
+// is 3.0.  Wider PSFs take the tap loop (convR), with the same sums.
 template <typename T, int NX> struct ConvHC {
-    static constexpr int cap = sizeof(T) == 8 ? PM_HC : 9;
+    static constexpr int cap = sizeof(T) == 8 ? PM_HC : 8;
     static constexpr int value = NX / 2 - 1 < cap ? NX / 2 - 1 : cap;
 };
 
