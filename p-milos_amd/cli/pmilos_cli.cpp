@@ -10,9 +10,11 @@
 //                           directory (t1 = *) and, with t2 = *, new cubes
 //                           as they appear; subfield, mask, output prefix
 //
-// Options: --mixed (FP32 spectra/RFs, FP64 solve), --autotune (pick the
-// kernel shape on the first cube), --watch-timeout S (default 600, the
-// reference's TIMEOUT_FILE), --watch-poll S (default 5).
+// Options: --mixed (FP32 spectra/RFs, FP64 solve), --mixed-toplim T (the
+// mixed form's chi2-unchanged threshold, default PM_MIXED_TOPLIM; 0 = the
+// reference's toplim test), --autotune (pick the kernel shape on the first
+// cube), --watch-timeout S (default 600, the reference's TIMEOUT_FILE),
+// --watch-poll S (default 5).
 //
 // Inputs, outputs and file names follow the reference; the differences
 // (all deliberate) are listed in usage().  Every pixel is inverted by
@@ -41,6 +43,7 @@ namespace {
 
 struct Opts {
     int precision = PM_FP64;
+    double mixed_toplim = PM_MIXED_TOPLIM;
     bool autotune = false;
     double watch_timeout = 600.0;
     double watch_poll = 5.0;
@@ -49,13 +52,17 @@ struct Opts {
 void usage()
 {
     fprintf(stderr,
-            "usage: pmilos_gpu <file.mtrol | file.minit> [--mixed] [--autotune]\n"
-            "                  [--watch-timeout S] [--watch-poll S]\n"
+            "usage: pmilos_gpu <file.mtrol | file.minit> [--mixed] [--mixed-toplim T]\n"
+            "                  [--autotune] [--watch-timeout S] [--watch-poll S]\n"
+            "  --mixed-toplim T: the mixed form ends its LM loop when chi2 changes by\n"
+            "  less than T/100 relative (default %g; 0 = the reference's toplim test\n"
+            "  alone, as milos.x)\n"
             "Deliberate differences from milos.x / pmilos.x:\n"
             "  FITS cubes in any axis order are read by their CTYPEs (the reference's\n"
             "  fast path scrambles HPLN,HPLT,WAVE,STOKES cubes); a blank output prefix\n"
             "  is no prefix (not \"!\"); PSF samples outside a .psf file's range are 0;\n"
-            "  the directory watch skips every name containing _mod or _stokes.\n");
+            "  the directory watch skips every name containing _mod or _stokes.\n",
+            PM_MIXED_TOPLIM);
 }
 
 std::string no_ext(const std::string &p)
@@ -211,6 +218,11 @@ pm_ctx *make_ctx(const Setup &s, const Opts &o, const float *slight, bool classi
     int rc = pm_ctx_create(&cfg, &ctx);
     if (rc) {
         fail("context creation failed", pm_strerror(rc));
+        return nullptr;
+    }
+    if (o.precision == PM_MIXED && (rc = pm_ctx_set_mixed_toplim(ctx, o.mixed_toplim))) {
+        fail("--mixed-toplim", pm_strerror(rc));
+        pm_ctx_destroy(ctx);
         return nullptr;
     }
     return ctx;
@@ -613,6 +625,7 @@ int main(int argc, char **argv)
     for (int i = 1; i < argc; i++) {
         std::string a = argv[i];
         if (a == "--mixed") o.precision = PM_MIXED;
+        else if (a == "--mixed-toplim" && i + 1 < argc) o.mixed_toplim = atof(argv[++i]);
         else if (a == "--autotune") o.autotune = true;
         else if (a == "--watch-timeout" && i + 1 < argc) o.watch_timeout = atof(argv[++i]);
         else if (a == "--watch-poll" && i + 1 < argc) o.watch_poll = atof(argv[++i]);

@@ -161,6 +161,27 @@ def test_cli_fits_inversion_and_estimates(work):
                                    o[[1, 2, 5, 6, 7]].astype(np.float32), rtol=1e-6, atol=1e-5)
 
 
+def test_cli_mixed_and_its_exit_threshold(work):
+    """--mixed inverts with the mixed form and its default exit threshold;
+    --mixed-toplim 0 restores the reference's toplim test.  Each gives the
+    bits of the library called directly with the same setting."""
+    rows, cols = 6, 5
+    lam, G, ctx = _ctx_l30(precision=pmilos.PM_MIXED)
+    models = draw_models(rows * cols, 93)
+    cube = add_noise(ctx.synth(models), models, 1e-3, 94)
+    fitsutil.write(str(work / "obs.fits"), fitsutil.stokes_cube(cube, rows, cols),
+                   ctypes=("WAVE-GRI", "STOKES", "HPLN-TAN", "HPLT-TAN"))
+    mtrol(str(work / "f.mtrol"), 50, "obs.fits", "l30.grid", psf=60)
+    for args, tl in ((["--mixed"], pmilos.PM_MIXED_TOPLIM), (["--mixed", "--mixed-toplim", "0"], 0.0)):
+        run(["f.mtrol"] + args, work)
+        ctx.set_mixed_toplim(tl)
+        ref = ctx.invert(cube)
+        planes = fitsutil.read(str(work / "obs_mod.fits"))[0].reshape(13, rows * cols)
+        np.testing.assert_array_equal(planes[:11].T, ref["model"].astype(np.float32))
+        np.testing.assert_array_equal(planes[11], ref["iters"])
+        say("cli %s: mean iters %.2f" % (" ".join(args), planes[11].mean()))
+
+
 def _minit(path, control, prefix, mask="", t1=1, t2=2, sub=(0, 0, 0, 0), save=1):
     lines = open(os.path.join(GOLDEN, "pmilos.minit")).read().splitlines()
     vals = [control, "fits", "fits", sub[0], sub[1], sub[2], sub[3], prefix, mask, t1, t2, save]
