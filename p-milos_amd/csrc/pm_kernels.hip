@@ -1693,14 +1693,14 @@ extern "C" int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3])
 extern "C" int pm_ctx_set_mixed_toplim(pm_ctx *ctx, double toplim_mixed)
 {
     if (!ctx || !(toplim_mixed >= 0.0) || !(toplim_mixed < 100.0)) return PM_EINVAL;
-    ctx->h.toplim_mixed = toplim_mixed;
     int dev = 0;
     PM_HIP(hipGetDevice(&dev));
     PM_HIP(hipSetDevice(ctx->device));
-    const hipError_t e = hipMemcpy(&ctx->d->toplim_mixed, &ctx->h.toplim_mixed, sizeof(double),
+    const hipError_t e = hipMemcpy(&ctx->d->toplim_mixed, &toplim_mixed, sizeof(double),
                                    hipMemcpyHostToDevice);
     PM_HIP(hipSetDevice(dev));
     PM_HIP(e);
+    ctx->h.toplim_mixed = toplim_mixed;       // the host copy follows the device's
     return PM_OK;
 }
 
