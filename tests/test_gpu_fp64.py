@@ -128,12 +128,13 @@ def _compare_inversions(g, o, label, tol=TOL64, frac=0.995):
     np.testing.assert_allclose(g["chi2"][both], o["chi2"][both], rtol=1e-4)
 
 
-def test_invert_fp64_config1_sample():
-    """Config-1 recipe (seed 1, sigma 1e-3 Ic, PSF 60 mA, 30 lambda, 9 free,
-    <= 50 iterations) on 1024 pixels, both from the same float32 cube."""
-    say("case invert L=30 npix=1024 seed=1")
+def test_invert_fp64_config1_field():
+    """The CPU-runnable field: 64x64 pixels of seed-1 models, sigma 1e-3 Ic,
+    PSF 60 mA, 30 lambda, 9 free, <= 50 iterations; the GPU and the oracle
+    invert the same float32 cube."""
+    say("case invert L=30 npix=4096 seed=1")
     s, ctx = _pair(30)
-    truths = draw_models(1024, seed=1)
+    truths = draw_models(64 * 64, seed=1)
     cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=11)
     o = O.invert(s, cube, GUESS)
     g = ctx.invert(cube, want_fit=True)
