@@ -7,6 +7,13 @@
  * process.  A context is used by one host thread at a time; its kernels run
  * asynchronously on the stream given to the *_device entry points (NULL =
  * the default stream), the host-pointer entry points are synchronous.
+ * Inversions of one context share its work-queue counters: one issued on a
+ * stream other than the previous one's waits for that one to finish (an
+ * event dependency, no host wait), so a context's inversions never overlap
+ * on the device; use one context per stream for concurrent inversions, or
+ * pm_graph / pm_server, which keep counters per branch and per stream.
+ * pm_invert_device / pm_invert_sl_device / pm_ctx_autotune refuse a stream
+ * under graph capture (PM_EINVAL): capture a batch with pm_graph_create.
  *
  * Reference interfaces replaced (P-MILOS, src/):
  *   pm_ctx_create    the per-run setup milos.c:133-263 + create_cuantic
@@ -189,8 +196,8 @@ int pm_ctx_set_variant(pm_ctx *ctx, int general, int index);
  * the reference's test only ends the loop once the damping has made the
  * step smaller than the FP32 rounding of every sample.  Mixed contexts start
  * with PM_MIXED_TOPLIM (|dchi2| / chi2 < 1e-6); 0 restores the reference's
- * test.  The FP64 form ignores it.  Call it while no inversion of this
- * context is running (it rewrites the context's device constants).
+ * test.  The FP64 form ignores it.  It waits for the context's last
+ * inversion to finish before it rewrites the context's device constants.
  * 0 <= toplim_mixed < 100, else PM_EINVAL. */
 #define PM_MIXED_TOPLIM 1e-4
 int pm_ctx_set_mixed_toplim(pm_ctx *ctx, double toplim_mixed);

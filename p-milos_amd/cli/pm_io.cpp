@@ -603,6 +603,11 @@ int read_cube(const char *path, int L, int x1, int x2, int y1, int y2, float **o
     bool ok = read_floats(f, total, raw);
     fclose(f);
     if (!ok) return -1;
+    // the physical value is BZERO + BSCALE * stored, formed in double and
+    // rounded to float, as a TFLOAT read through cfitsio returns it
+    // (fits_read_pix, utilsFits.c:212)
+    if (h.bscale != 1.0 || h.bzero != 0.0)
+        for (size_t i = 0; i < total; i++) raw[i] = (float)(h.bzero + h.bscale * (double)raw[i]);
     long r0 = 0, r1 = rows, c0 = 0, c1 = cols;
     if (x1 || x2 || y1 || y2) {
         if (x1 < 1 || y1 < 1 || x2 > rows || y2 > cols || x1 > x2 || y1 > y2) return -5;

@@ -1181,7 +1181,34 @@ struct pm_ctx {
     int ncu = 0;
     hipStream_t last_stream = nullptr;
     int variant[2] = {0, 0};   // kernel variant of the 9-row and general forms
+    // recorded after every launch that uses the context's work-queue
+    // counters, so that a launch on another stream waits for it (the
+    // counters are reset at the start of each launch)
+    hipEvent_t done = nullptr;
+    bool launched = false;
 };
+
+// Orders a launch that uses the context's counters after the previous one
+// when the stream changes (outside graph capture, where the graph's own
+// counters are used).
+static int ctx_counters_acquire(pm_ctx *ctx, hipStream_t st)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(st, &cs) != hipSuccess) return PM_EDEVICE;
+    if (cs != hipStreamCaptureStatusNone) return PM_EINVAL;
+    if (ctx->launched && st != ctx->last_stream &&
+        hipStreamWaitEvent(st, ctx->done, 0) != hipSuccess)
+        return PM_EDEVICE;
+    return PM_OK;
+}
+
+static int ctx_counters_release(pm_ctx *ctx, hipStream_t st)
+{
+    ctx->last_stream = st;
+    if (hipEventRecord(ctx->done, st) != hipSuccess) return PM_EDEVICE;
+    ctx->launched = true;
+    return PM_OK;
+}
 
 extern "C" const char *pm_strerror(int status)
 {
@@ -1392,6 +1419,10 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
         pm_ctx_destroy(ctx);
         return PM_ENOMEM;
     }
+    if (hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) {
+        pm_ctx_destroy(ctx);
+        return PM_EDEVICE;
+    }
     if (hipMemcpy(ctx->d, &c, sizeof(PmConst), hipMemcpyHostToDevice) != hipSuccess) {
         pm_ctx_destroy(ctx);
         return PM_EDEVICE;
@@ -1406,6 +1437,7 @@ extern "C" void pm_ctx_destroy(pm_ctx *ctx)
     if (ctx->d) (void)hipFree(ctx->d);
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->fallbacks) (void)hipFree(ctx->fallbacks);
+    if (ctx->done) (void)hipEventDestroy(ctx->done);
     delete ctx;
 }
 
@@ -1528,7 +1560,6 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     if (!ctx || n < 0 || (n > 0 && (!d_models || !d_spectra))) return PM_EINVAL;
     if (n == 0) return PM_OK;
     hipStream_t st = (hipStream_t)stream;
-    ctx->last_stream = st;
     const bool gen = needs_general(ctx, false);
     InvVariant v[PM_MAXVAR];
     const int nv = variants(ctx, gen, v);
@@ -1584,10 +1615,14 @@ extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const flo
                                    int *d_accepted, float *d_fit, void *stream)
 {
     if (!ctx) return PM_EINVAL;
+    if (npix == 0) return PM_OK;
     hipStream_t st = (hipStream_t)stream;
-    ctx->last_stream = st;
-    return pm_invert_launch(ctx, d_stokes, d_slight, npix, d_models, d_chi2, d_iters, d_accepted,
-                            d_fit, ctx->counters, ctx->fallbacks, st, 1.0);
+    int rc = ctx_counters_acquire(ctx, st);
+    if (rc != PM_OK) return rc;
+    rc = pm_invert_launch(ctx, d_stokes, d_slight, npix, d_models, d_chi2, d_iters, d_accepted,
+                          d_fit, ctx->counters, ctx->fallbacks, st, 1.0);
+    if (rc != PM_OK) return rc;
+    return ctx_counters_release(ctx, st);
 }
 
 extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
@@ -1605,7 +1640,6 @@ extern "C" int pm_classical_device(pm_ctx *ctx, const float *d_stokes, int npix,
     if (npix == 0) return PM_OK;
     if (!d_stokes || !d_models) return PM_EINVAL;
     hipStream_t st = (hipStream_t)stream;
-    ctx->last_stream = st;
     hipLaunchKernelGGL(pm_classical_kernel, dim3((unsigned)((npix + 127) / 128)), dim3(128), 0, st,
                        (const PmConst *)ctx->d, d_stokes, npix, d_models);
     PM_HIP(hipGetLastError());
@@ -1616,7 +1650,7 @@ extern "C" int pm_last_fallback_count(pm_ctx *ctx, long long *count)
 {
     if (!ctx || !count) return PM_EINVAL;
     unsigned long long v = 0;
-    PM_HIP(hipStreamSynchronize(ctx->last_stream));
+    if (ctx->launched) PM_HIP(hipEventSynchronize(ctx->done));
     PM_HIP(hipMemcpy(&v, ctx->fallbacks, sizeof(v), hipMemcpyDeviceToHost));
     *count = (long long)v;
     return PM_OK;
@@ -1629,6 +1663,10 @@ extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *
 {
     if (!ctx || !d_stokes || npix < 1) return PM_EINVAL;
     hipStream_t st = (hipStream_t)stream;
+    {
+        const int rc0 = ctx_counters_acquire(ctx, st);
+        if (rc0 != PM_OK) return rc0;
+    }
     const bool gen = needs_general(ctx, d_slight != nullptr);
     InvVariant v[PM_MAXVAR];
     const int nv = variants(ctx, gen, v);
@@ -1670,6 +1708,7 @@ extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
+    if (ctx_counters_release(ctx, st) != PM_OK && rc == PM_OK) rc = PM_EDEVICE;
     (void)hipStreamSynchronize(st);
     (void)hipFree(buf);
     if (rc != PM_OK) return rc;
@@ -1696,8 +1735,12 @@ extern "C" int pm_ctx_set_mixed_toplim(pm_ctx *ctx, double toplim_mixed)
     int dev = 0;
     PM_HIP(hipGetDevice(&dev));
     PM_HIP(hipSetDevice(ctx->device));
-    const hipError_t e = hipMemcpy(&ctx->d->toplim_mixed, &toplim_mixed, sizeof(double),
-                                   hipMemcpyHostToDevice);
+    // the context's last kernels read the constants: let them finish first
+    // (the copy below is on the null stream, which does not wait for
+    // non-blocking streams)
+    hipError_t e = ctx->launched ? hipEventSynchronize(ctx->done) : hipSuccess;
+    if (e == hipSuccess)
+        e = hipMemcpy(&ctx->d->toplim_mixed, &toplim_mixed, sizeof(double), hipMemcpyHostToDevice);
     PM_HIP(hipSetDevice(dev));
     PM_HIP(e);
     ctx->h.toplim_mixed = toplim_mixed;       // the host copy follows the device's

@@ -62,23 +62,8 @@ def build(verbose=False):
     if verbose and errs:
         print("\n".join(errs))
     os.replace(tmp, LIB)
-    build_probe(hipcc, verbose)
     build_cli(verbose)
     return LIB
-
-
-def build_probe(hipcc, verbose=False):
-    """libpm_jtj_probe.so: J^T J on the matrix cores against the vector form,
-    for pmilos.jtj_probe."""
-    out = os.path.join(HERE, "libpm_jtj_probe.so")
-    cmd = [hipcc] + HIPCC_FLAGS + ["-o", out + ".tmp", os.path.join(CSRC, "pm_jtj_probe.hip")]
-    if verbose:
-        print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stderr)
-        raise RuntimeError("hipcc failed (%d)" % res.returncode)
-    os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

@@ -16,7 +16,7 @@ def _card(key, value):
     return c.ljust(80)[:80]
 
 
-def write(path, data, ctypes=None, bitpix=-32, extra=(), bzero=None):
+def write(path, data, ctypes=None, bitpix=-32, extra=(), bzero=None, bscale=1):
     """data: numpy array in C order, whose LAST axis is NAXIS1 (FITS order
     reversed, as numpy stores it)."""
     data = np.asarray(data)
@@ -26,7 +26,7 @@ def write(path, data, ctypes=None, bitpix=-32, extra=(), bzero=None):
     if ctypes:
         cards += [_card("CTYPE%d" % (k + 1), t) for k, t in enumerate(ctypes)]
     if bzero is not None:
-        cards += [_card("BSCALE", 1), _card("BZERO", bzero)]
+        cards += [_card("BSCALE", bscale), _card("BZERO", bzero)]
     cards += [c.ljust(80)[:80] for c in extra]
     cards.append("END".ljust(80))
     hdr = "".join(cards)

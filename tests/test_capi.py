@@ -67,16 +67,3 @@ def test_error_codes_without_device_calls(lib):
     lib.pm_ctx_set_mixed_toplim.argtypes = [ctypes.c_void_p, ctypes.c_double]
     assert lib.pm_ctx_set_mixed_toplim(None, 1e-3) == -22
 
-
-def test_jtj_probe_library_rows():
-    """The J^T J probe library loads without a device, and the rows it
-    generates are bounded, zero beyond 30 wavelengths, differ per pixel."""
-    from pmilos import jtj_probe
-    L = jtj_probe._lib()
-    r = np.zeros((10, 4, 32))
-    L.pmp_jtj_rows(5, 3, r.ctypes.data)
-    assert np.all(np.abs(r) <= 0.5) and np.all(r[:, :, 30:] == 0)
-    assert np.count_nonzero(r[:, :, :30]) > 1000
-    r2 = np.zeros_like(r)
-    L.pmp_jtj_rows(6, 3, r2.ctypes.data)
-    assert not np.array_equal(r, r2)

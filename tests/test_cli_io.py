@@ -240,6 +240,21 @@ def test_fits_cube_without_ctypes_and_errors(io, tmp_path):
     assert _read_cube(io, p, L)[0] == -2
 
 
+def test_fits_cube_scaled_float(io, tmp_path):
+    """A BITPIX -32 cube with BSCALE/BZERO is read as BZERO + BSCALE * stored,
+    rounded to float (what cfitsio's TFLOAT read returns, utilsFits.c:212)."""
+    rows, cols, L = 4, 3, 6
+    stored = np.random.default_rng(8).normal(size=(rows * cols, 4, L)).astype(np.float32)
+    p = str(tmp_path / "s.fits")
+    fitsutil.write(p, fitsutil.stokes_cube(stored, rows, cols),
+                   ctypes=("WAVE-GRI", "STOKES", "HPLN-TAN", "HPLT-TAN"), bzero=0.25, bscale=2.5)
+    rc, got, lay = _read_cube(io, p, L)
+    assert rc == 0
+    want = (0.25 + 2.5 * stored.astype(np.float64)).astype(np.float32)
+    assert np.array_equal(got, want)
+    io.pmio_free_layout(C.byref(lay))
+
+
 def test_fits_models_planes(io, tmp_path):
     rows, cols = 6, 4
     n = rows * cols
