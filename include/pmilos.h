@@ -52,7 +52,8 @@ enum pm_status {
     PM_EINVAL = -22,      /* bad argument */
     PM_ENOMEM = -12,      /* device allocation failed */
     PM_EDEVICE = -5,      /* HIP runtime error */
-    PM_ENOTSUP = -95      /* configuration not supported by the kernels */
+    PM_ENOTSUP = -95,     /* configuration not supported by the kernels */
+    PM_ERANGE = -34       /* size above the entry point's maximum */
 };
 
 enum pm_precision {
@@ -138,6 +139,26 @@ int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models,
 int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
                      float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
                      void *stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): pm_invert_device without the
+ * context's work-queue counters.  It allocates nothing and calls no
+ * allocating routine; its counters live in a fixed device scratch array of
+ * the library (1,088 bytes per GPU, held until the process ends) whose
+ * address is looked up once per device by pm_ctx_create.  The counters are
+ * cleared by a kernel at the start of every call (no memset), so the call
+ * can be captured into a graph.  Every call on one GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two streams, or two graph
+ * replays in flight) race, so order them.  The current device must be the
+ * context's (PM_EINVAL otherwise); npix > PM_NOALLOC_MAX_NPIX gives
+ * PM_ERANGE (the work-queue counters are 32-bit).  The eigen-fallback count
+ * of these calls is not reported by pm_last_fallback_count.  Otherwise as
+ * pm_invert_device (the same kernels, the same results). */
+#define PM_NOALLOC_MAX_NPIX (1 << 30)
+int pm_invert_device_noalloc(pm_ctx *ctx, const float *d_stokes, int npix, double *d_models,
+                             float *d_chi2, int *d_iters, int *d_accepted, float *d_fit,
+                             void *stream);
 
 /* Inversion with a stray-light profile per pixel: slight [npix][4][L] float
  * (the 4-D stray-light cube of readFitsStrayLightFile, utilsFits.c:881,

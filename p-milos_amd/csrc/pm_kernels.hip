@@ -15,6 +15,7 @@
 #include <stdio.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 #include <stdlib.h>
 #include <string.h>
@@ -1186,6 +1187,7 @@ struct pm_ctx {
     // counters are reset at the start of each launch)
     hipEvent_t done = nullptr;
     bool launched = false;
+    void *scratch = nullptr;   // this device's scratch (pm_invert_device_noalloc)
 };
 
 // Orders a launch that uses the context's counters after the previous one
@@ -1210,6 +1212,38 @@ static int ctx_counters_release(pm_ctx *ctx, hipStream_t st)
     return PM_OK;
 }
 
+// ------------------------------------------------------------ no-allocation scratch
+
+// Device scratch of pm_invert_device_noalloc: the work-queue counters
+// (PM_NSLICE counters PM_CSTRIDE ints apart) followed by the fallback
+// counter.  Each GPU holds its own copy for the life of the process.  In
+// host code the name stands for no device address: the address of this
+// GPU's copy comes from hipGetSymbolAddress (scratch_of).
+#define PM_SCRATCH_WORDS 136             // 8-byte words: 1,088 bytes per GPU
+static_assert(PM_NSLICE * PM_CSTRIDE * sizeof(int) + sizeof(unsigned long long) <=
+                  PM_SCRATCH_WORDS * sizeof(unsigned long long),
+              "scratch too small for the counters");
+static __device__ __align__(256) unsigned long long scratch[PM_SCRATCH_WORDS];
+
+#define PM_MAX_DEVICES 64
+static std::mutex scratch_mu;
+static void *scratch_addr[PM_MAX_DEVICES];
+
+// The device address of the current device's scratch, looked up once per
+// device (with that device current) and kept.
+static int scratch_of(int dev, void **out)
+{
+    if (dev < 0 || dev >= PM_MAX_DEVICES) return PM_ENOTSUP;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_addr[dev]) {
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return PM_EDEVICE;
+        scratch_addr[dev] = p;
+    }
+    *out = scratch_addr[dev];
+    return PM_OK;
+}
+
 extern "C" const char *pm_strerror(int status)
 {
     switch (status) {
@@ -1218,6 +1252,7 @@ extern "C" const char *pm_strerror(int status)
     case PM_ENOMEM: return "device allocation failed";
     case PM_EDEVICE: return "HIP runtime error";
     case PM_ENOTSUP: return "configuration not supported";
+    case PM_ERANGE: return "size above the entry point's maximum";
     default: return "unknown error";
     }
 }
@@ -1422,6 +1457,10 @@ extern "C" int pm_ctx_create(const pm_config *cfg, pm_ctx **out)
     if (hipEventCreateWithFlags(&ctx->done, hipEventDisableTiming) != hipSuccess) {
         pm_ctx_destroy(ctx);
         return PM_EDEVICE;
+    }
+    if ((rc = scratch_of(ctx->device, &ctx->scratch)) != PM_OK) {
+        pm_ctx_destroy(ctx);
+        return rc;
     }
     if (hipMemcpy(ctx->d, &c, sizeof(PmConst), hipMemcpyHostToDevice) != hipSuccess) {
         pm_ctx_destroy(ctx);
@@ -1631,6 +1670,25 @@ extern "C" int pm_invert_device(pm_ctx *ctx, const float *d_stokes, int npix, do
 {
     return pm_invert_sl_device(ctx, d_stokes, nullptr, npix, d_models, d_chi2, d_iters,
                                d_accepted, d_fit, stream);
+}
+
+extern "C" int pm_invert_device_noalloc(pm_ctx *ctx, const float *d_stokes, int npix,
+                                        double *d_models, float *d_chi2, int *d_iters,
+                                        int *d_accepted, float *d_fit, void *stream)
+{
+    if (!ctx || npix < 0) return PM_EINVAL;
+    if (npix > PM_NOALLOC_MAX_NPIX) return PM_ERANGE;
+    if (npix == 0) return PM_OK;
+    if (!ctx->scratch) return PM_EDEVICE;
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return PM_EDEVICE;
+    if (dev != ctx->device) return PM_EINVAL;
+    // counters first, the fallback counter after them; both cleared by the
+    // launch's reset kernel (no memset, so the launch can be captured)
+    int *cnt = (int *)ctx->scratch;
+    unsigned long long *fb = (unsigned long long *)ctx->scratch + PM_NSLICE * PM_CSTRIDE / 2;
+    return pm_invert_launch(ctx, d_stokes, nullptr, npix, d_models, d_chi2, d_iters, d_accepted,
+                            d_fit, cnt, fb, (hipStream_t)stream, 1.0);
 }
 
 extern "C" int pm_classical_device(pm_ctx *ctx, const float *d_stokes, int npix,

@@ -73,6 +73,7 @@ def lib():
         L.pm_synth_device.argtypes = [vp, vp, C.c_int, vp, vp, vp]
         L.pm_invert.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp]
         L.pm_invert_device.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
+        L.pm_invert_device_noalloc.argtypes = [vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.pm_invert_sl.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp]
         L.pm_invert_sl_device.argtypes = [vp, vp, vp, C.c_int, vp, vp, vp, vp, vp, vp]
         L.pm_last_fallback_count.argtypes = [vp, C.POINTER(C.c_longlong)]
@@ -206,6 +207,14 @@ class Context:
         else:
             _check(lib().pm_invert_device(self.h, d_stokes, npix, d_models, d_chi2, d_iters,
                                           d_accepted or None, d_fit or None, stream or None))
+
+    def invert_ptrs_noalloc(self, d_stokes, npix, d_models, d_chi2, d_iters, d_accepted=0,
+                            d_fit=0, stream=0):
+        """pm_invert_device_noalloc: as invert_ptrs, with the library's fixed
+        device scratch instead of the context's counters (allocates nothing;
+        calls overlapping on one GPU race)."""
+        _check(lib().pm_invert_device_noalloc(self.h, d_stokes, npix, d_models, d_chi2, d_iters,
+                                              d_accepted or None, d_fit or None, stream or None))
 
     def synth_ptrs(self, d_models, n, d_spectra, d_rf=0, stream=0):
         _check(lib().pm_synth_device(self.h, d_models, n, d_spectra, d_rf or None,
