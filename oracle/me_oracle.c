@@ -32,6 +32,21 @@
 #include <stdlib.h>
 #include <string.h>
 
+/* The arithmetic type of what the reference declares REAL (defines.h:8-36):
+ * double by default (USE_DOUBLE_PRECISION); float when built with
+ * -DORC_REAL=float, the reference's default build (makefile:21-25), in which
+ * PRECISION quantities (the model, lambda, the PSF and macroturbulence
+ * kernels, the convolution sums, the solve) stay double and the REAL
+ * trigonometric macros are the float functions. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef ORC_REAL real;
+#define ORC_REAL_IS_FLOAT (sizeof(real) == sizeof(float))
+static inline real r_sin(real x) { return ORC_REAL_IS_FLOAT ? (real)sinf((float)x) : (real)sin(x); }
+static inline real r_cos(real x) { return ORC_REAL_IS_FLOAT ? (real)cosf((float)x) : (real)cos(x); }
+static inline real r_fabs(real x) { return ORC_REAL_IS_FLOAT ? (real)fabsf((float)x) : (real)fabs(x); }
+
 /* Constants, defines.h:66-78. */
 #define ORC_PI 3.14159265358979323846264338327950288419716939937510
 #define ORC_RR 0.5641895836
@@ -168,6 +183,66 @@ void orc_fvoigt(double damp, const double *v, int n, double *H, double *F)
     }
 }
 
+/* orc_fvoigt on REAL arguments (fvoigt.c:15: vv, h, f are REAL, the
+ * rational function is evaluated in complex double). */
+static void fvoigt_r(double damp, const real *v, int n, real *H, real *F)
+{
+    for (int i = 0; i < n; i++) {
+        double complex z = damp - v[i] * I;
+        double complex num = VA[6];
+        for (int j = 5; j >= 0; j--) num = num * z + VA[j];
+        double complex den = z + VB[6];
+        for (int j = 5; j >= 0; j--) den = den * z + VB[j];
+        double complex w = num / den;
+        H[i] = creal(w);
+        F[i] = cimag(w) * 0.5;
+    }
+}
+
+/* The correlations of convolution.c on REAL rows (:54-78, :90-113) and the
+ * circular convolution (:126-153): PRECISION kernel and sums, REAL data. */
+static void conv_direct_r(real *x, int nx, const double *h, int nh)
+{
+    int half = nh / 2;
+    double pad[3 * ORC_MAXL + 2];
+    memset(pad, 0, sizeof(double) * (size_t)(nx + nh + 1));
+    for (int k = 0; k < nx; k++) pad[k + half] = x[k];
+    for (int k = 0; k < nx; k++) {
+        double acc = 0;
+        for (int j = 0; j < nh; j++) acc += h[j] * pad[j + k];
+        x[k] = acc;
+    }
+}
+
+static void conv_direct_ic_r(real *x, int nx, const double *h, int nh, real ic)
+{
+    int half = nh / 2;
+    double pad[3 * ORC_MAXL + 2];
+    memset(pad, 0, sizeof(double) * (size_t)(nx + nh + 1));
+    for (int k = 0; k < nx; k++) pad[k + half] = ic - x[k];
+    for (int k = 0; k < nx; k++) {
+        double acc = 0;
+        for (int j = 0; j < nh; j++) acc += h[j] * pad[j + k];
+        x[k] = ic - acc;
+    }
+}
+
+static void conv_circular_r(const real *x, const double *h, int n, real *out)
+{
+    real res[ORC_MAXL];
+    for (int i = 0; i < n; i++) {
+        double acc = 0;
+        for (int j = 0; j < n; j++) {
+            int m = i - j;
+            if (m < 0) m += n;
+            acc += h[j] * x[m];
+        }
+        int dst = (i < n / 2) ? (i + n / 2 + (n % 2)) : (i - n / 2);
+        res[dst] = acc;
+    }
+    memcpy(out, res, sizeof(real) * (size_t)n);
+}
+
 /* Macroturbulence Gaussian (and its derivative w.r.t. v_mac), centred on
  * lambda[n/2], unit sum.  fgauss.c:16-66. */
 void orc_gauss_mac(double mc, const double *lambda, int n, double lambda_line,
@@ -247,48 +322,51 @@ void orc_conv_circular(const double *x, const double *h, int n, double *out)
  * in globals, calculosCompartidos.c:34-154). */
 typedef struct {
     int L, ncomp[3];
-    double sin_gm, cosi, sinis, cosis, cosis_2, sina, cosa;
-    double sinda, cosda, sindi, cosdi, sinis_cosa, sinis_sina;
-    double E0;
-    double shift[3][ORC_MAXC];                  /* nupB, nubB, nurB */
-    double uu[3][ORC_MAXC][ORC_MAXL];
-    double H[3][ORC_MAXC][ORC_MAXL], F[3][ORC_MAXC][ORC_MAXL];
-    double fi[3][ORC_MAXL], shi[3][ORC_MAXL];   /* p, b, r */
-    double parcial1[ORC_MAXL], parcial2[ORC_MAXL], parcial3[ORC_MAXL];
-    double etain[ORC_MAXL], etaqn[ORC_MAXL], etaun[ORC_MAXL], etavn[ORC_MAXL];
-    double rhoqn[ORC_MAXL], rhoun[ORC_MAXL], rhovn[ORC_MAXL];
-    double etai[ORC_MAXL], etaq[ORC_MAXL], etau[ORC_MAXL], etav[ORC_MAXL];
-    double rhoq[ORC_MAXL], rhou[ORC_MAXL], rhov[ORC_MAXL];
-    double etai_2[ORC_MAXL], gp1[ORC_MAXL], gp2[ORC_MAXL], gp3[ORC_MAXL];
-    double dt[ORC_MAXL], dti[ORC_MAXL];
-    double g4r[ORC_MAXL], g5r[ORC_MAXL], g6r[ORC_MAXL];  /* gpX_gp2_rhoY */
-    double spectra_mac[4 * ORC_MAXL];           /* before convolution */
-    double spectra_slight[4 * ORC_MAXL];        /* before stray-light mix */
-    double d_ei[7][ORC_MAXL], d_eq[7][ORC_MAXL], d_eu[7][ORC_MAXL], d_ev[7][ORC_MAXL];
-    double d_rq[7][ORC_MAXL], d_ru[7][ORC_MAXL], d_rv[7][ORC_MAXL];
+    real sin_gm, cosi, sinis, cosis, cosis_2, sina, cosa;
+    real sinda, cosda, sindi, cosdi, sinis_cosa, sinis_sina;
+    double E0;                                  /* PRECISION */
+    real shift[3][ORC_MAXC];                  /* nupB, nubB, nurB */
+    real uu[3][ORC_MAXC][ORC_MAXL];
+    real H[3][ORC_MAXC][ORC_MAXL], F[3][ORC_MAXC][ORC_MAXL];
+    real fi[3][ORC_MAXL], shi[3][ORC_MAXL];   /* p, b, r */
+    real parcial1[ORC_MAXL], parcial2[ORC_MAXL], parcial3[ORC_MAXL];
+    real etain[ORC_MAXL], etaqn[ORC_MAXL], etaun[ORC_MAXL], etavn[ORC_MAXL];
+    real rhoqn[ORC_MAXL], rhoun[ORC_MAXL], rhovn[ORC_MAXL];
+    real etai[ORC_MAXL], etaq[ORC_MAXL], etau[ORC_MAXL], etav[ORC_MAXL];
+    real rhoq[ORC_MAXL], rhou[ORC_MAXL], rhov[ORC_MAXL];
+    real etai_2[ORC_MAXL], gp1[ORC_MAXL], gp2[ORC_MAXL], gp3[ORC_MAXL];
+    real dt[ORC_MAXL], dti[ORC_MAXL];
+    real g4r[ORC_MAXL], g5r[ORC_MAXL], g6r[ORC_MAXL];  /* gpX_gp2_rhoY */
+    real spectra_mac[4 * ORC_MAXL];           /* before convolution */
+    real spectra_slight[4 * ORC_MAXL];        /* before stray-light mix */
+    real d_ei[7][ORC_MAXL], d_eq[7][ORC_MAXL], d_eu[7][ORC_MAXL], d_ev[7][ORC_MAXL];
+    real d_rq[7][ORC_MAXL], d_ru[7][ORC_MAXL], d_rv[7][ORC_MAXL];
 } orc_ws_t;
 
-/* mil_sinrf.c:46-415.  Writes spectra[4L] and fills ws for orc_rf. */
-static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
+/* mil_sinrf.c:46-415.  Writes spectra[4L] and fills ws for orc_rf.  The
+ * model (Init_Model) and its locals are PRECISION; the angle terms, the
+ * Zeeman splittings and weights, u and every per-wavelength quantity are
+ * REAL (milos.c:47-73, mil_sinrf.c:54-61). */
+static void synth_ws(const orc_setup_t *s, const double *m, real *spectra,
                      orc_ws_t *ws)
 {
     const int L = s->nlambda;
     const orc_zeeman_t *z = &s->z;
-    const double CC = ORC_PI / 180.0, CC_2 = CC * 2;              /* milos.c:232-233 */
+    const real CC = ORC_PI / 180.0, CC_2 = CC * 2;                /* milos.c:232-233 */
     double MF = m[ORC_B], VL = m[ORC_VLOS], LD = m[ORC_DOPP], A = m[ORC_AA];
     double B0 = m[ORC_S0], B1 = -((m[ORC_S1]) * s->ah);            /* :71-72 */
     double MC = m[ORC_MAC], ALF = m[ORC_ALFA];
     double AZI = m[ORC_AZ] * CC, GM = m[ORC_GM] * CC;              /* :84-85 */
 
     ws->L = L;
-    ws->sin_gm = sin(GM);                                          /* :87-103 */
-    ws->cosi = cos(GM);
+    ws->sin_gm = r_sin(GM);                                        /* :87-103 */
+    ws->cosi = r_cos(GM);
     ws->sinis = ws->sin_gm * ws->sin_gm;
     ws->cosis = ws->cosi * ws->cosi;
     ws->cosis_2 = (1 + ws->cosis) / 2;
-    double azi_2 = 2 * AZI;
-    ws->sina = sin(azi_2);
-    ws->cosa = cos(azi_2);
+    real azi_2 = 2 * AZI;
+    ws->sina = r_sin(azi_2);
+    ws->cosa = r_cos(azi_2);
     ws->sinda = ws->cosa * CC_2;
     ws->cosda = -ws->sina * CC_2;
     ws->sindi = ws->cosi * ws->sin_gm * CC_2;
@@ -298,14 +376,14 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
 
     double E0 = m[ORC_ETA0] * z->fo;                               /* :117 */
     ws->E0 = E0;
-    double ulos = (VL * s->lambda0) / (ORC_VLIGHT * LD);           /* :120 */
-    double u[ORC_MAXL];
+    real ulos = (VL * s->lambda0) / (ORC_VLIGHT * LD);             /* :120 */
+    real u[ORC_MAXL];
     for (int i = 0; i < L; i++) u[i] = ((s->lambda[i] - s->lambda0) / LD) - ulos;
 
-    double parcial = ((s->lambda0 * s->lambda0) / LD) * ORC_CTE4_6_13; /* :153 */
+    real parcial = ((s->lambda0 * s->lambda0) / LD) * ORC_CTE4_6_13; /* :153 */
     ws->ncomp[0] = z->n_pi; ws->ncomp[1] = z->n_sig; ws->ncomp[2] = z->n_sig;
-    for (int i = 0; i < z->n_pi; i++) ws->shift[0][i] = parcial * z->nup[i];
-    for (int i = 0; i < z->n_sig; i++) ws->shift[1][i] = parcial * z->nub[i];
+    for (int i = 0; i < z->n_pi; i++) ws->shift[0][i] = parcial * (real)z->nup[i];
+    for (int i = 0; i < z->n_sig; i++) ws->shift[1][i] = parcial * (real)z->nub[i];
     for (int i = 0; i < z->n_sig; i++)                             /* :164-166 */
         ws->shift[2][i] = -ws->shift[1][z->n_sig - (i + 1)];
 
@@ -313,24 +391,25 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
     for (int g = 0; g < 3; g++) {           /* funcionComponentFor_sinrf :418-449 */
         for (int i = 0; i < L; i++) { ws->fi[g][i] = 0; ws->shi[g][i] = 0; }
         for (int c = 0; c < ws->ncomp[g]; c++) {
-            double *uu = ws->uu[g][c], *H = ws->H[g][c], *F = ws->F[g][c];
+            real *uu = ws->uu[g][c], *H = ws->H[g][c], *F = ws->F[g][c];
+            const real w = (real)wgt[g][c];
             for (int j = 0; j < L; j++) uu[j] = u[j] - ws->shift[g][c] * MF;
-            orc_fvoigt(A, uu, L, H, F);
-            for (int j = 0; j < L; j++) ws->fi[g][j] = ws->fi[g][j] + wgt[g][c] * H[j];
+            fvoigt_r(A, uu, L, H, F);
+            for (int j = 0; j < L; j++) ws->fi[g][j] = ws->fi[g][j] + w * H[j];
             for (int j = 0; j < L; j++)
-                ws->shi[g][j] = (ws->shi[g][j] + (wgt[g][c] * F[j] * 2));
+                ws->shi[g][j] = (ws->shi[g][j] + (w * F[j] * 2));
         }
     }
-    const double *fi_p = ws->fi[0], *fi_b = ws->fi[1], *fi_r = ws->fi[2];
-    const double *shi_p = ws->shi[0], *shi_b = ws->shi[1], *shi_r = ws->shi[2];
+    const real *fi_p = ws->fi[0], *fi_b = ws->fi[1], *fi_r = ws->fi[2];
+    const real *shi_p = ws->shi[0], *shi_b = ws->shi[1], *shi_r = ws->shi[2];
 
-    double E0_2 = E0 / 2.0;                                         /* :188-223 */
+    real E0_2 = E0 / 2.0;                                           /* :188-223 */
     for (int i = 0; i < L; i++) {
         ws->parcial1[i] = fi_b[i] + fi_r[i];
         ws->parcial2[i] = (E0_2) * (fi_p[i] - (ws->parcial1[i]) / 2);
         ws->parcial3[i] = (E0_2) * (shi_p[i] - (shi_b[i] + shi_r[i]) / 2);
     }
-    double cosi_E0_2 = E0_2 * ws->cosi;
+    real cosi_E0_2 = E0_2 * ws->cosi;
     for (int i = 0; i < L; i++) {
         ws->etain[i] = ((E0_2) * (fi_p[i] * ws->sinis + (ws->parcial1[i]) * ws->cosis_2));
         ws->etaqn[i] = (ws->parcial2[i] * ws->sinis_cosa);
@@ -345,31 +424,31 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
     }
 
     for (int i = 0; i < L; i++) {                                   /* :229-266 */
-        double ei = ws->etai[i], eq = ws->etaq[i], eu = ws->etau[i], ev = ws->etav[i];
-        double rq = ws->rhoq[i], ru = ws->rhou[i], rv = ws->rhov[i];
-        double ei2 = ei * ei;
-        double auxq = rq * rq, auxu = ru * ru, auxv = rv * rv;
+        real ei = ws->etai[i], eq = ws->etaq[i], eu = ws->etau[i], ev = ws->etav[i];
+        real rq = ws->rhoq[i], ru = ws->rhou[i], rv = ws->rhov[i];
+        real ei2 = ei * ei;
+        real auxq = rq * rq, auxu = ru * ru, auxv = rv * rv;
         ws->etai_2[i] = ei2;
         ws->gp1[i] = ei2 - eq * eq - eu * eu - ev * ev + auxq + auxu + auxv;
         ws->gp3[i] = ei2 + auxq + auxu + auxv;
         ws->gp2[i] = eq * rq + eu * ru + ev * rv;
         ws->dt[i] = ei2 * ws->gp1[i] - ws->gp2[i] * ws->gp2[i];
         ws->dti[i] = 1.0 / ws->dt[i];
-        double gp4 = ei2 * eq + ei * (ev * ru - eu * rv);
-        double gp5 = ei2 * eu + ei * (eq * rv - ev * rq);
-        double gp6 = (ei2) * ev + ei * (eu * rq - eq * ru);
+        real gp4 = ei2 * eq + ei * (ev * ru - eu * rv);
+        real gp5 = ei2 * eu + ei * (eq * rv - ev * rq);
+        real gp6 = (ei2) * ev + ei * (eu * rq - eq * ru);
         ws->g4r[i] = gp4 + rq * ws->gp2[i];
         ws->g5r[i] = gp5 + ru * ws->gp2[i];
         ws->g6r[i] = gp6 + rv * ws->gp2[i];
     }
     for (int i = 0; i < L; i++) {                                   /* :268-285 */
-        double dtiaux = ws->dti[i] * (B1);
+        real dtiaux = ws->dti[i] * (B1);
         spectra[i] = B0 - dtiaux * ws->etai[i] * ws->gp3[i];
         spectra[i + L] = (dtiaux * (ws->g4r[i]));
         spectra[i + 2 * L] = (dtiaux * (ws->g5r[i]));
         spectra[i + 3 * L] = (dtiaux * (ws->g6r[i]));
     }
-    memcpy(ws->spectra_mac, spectra, sizeof(double) * 4 * (size_t)L);
+    memcpy(ws->spectra_mac, spectra, sizeof(real) * 4 * (size_t)L);
 
     int mac_applied = 0;
     if (MC > 0.0001) {                                              /* :288-357 */
@@ -381,20 +460,20 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
         else if (s->use_psf)
             orc_conv_direct(gmac, L, s->G, L);
         for (int st = 0; st < 4; st++)
-            orc_conv_circular(spectra + L * st, gmac, L, spectra + L * st);
+            conv_circular_r(spectra + L * st, gmac, L, spectra + L * st);
     }
     if (!mac_applied && s->use_psf && s->use_fft) {                 /* :362-386 */
         for (int st = 0; st < 4; st++)
-            orc_conv_circular(spectra + L * st, s->G, L, spectra + L * st);
+            conv_circular_r(spectra + L * st, s->G, L, spectra + L * st);
     } else if (!mac_applied && s->use_psf) {                        /* :387-402 */
-        double ic = (spectra[0] > spectra[L - 1]) ? spectra[0] : spectra[L - 1];
-        orc_conv_direct_ic(spectra, L, s->G, L, ic);
-        for (int st = 1; st < 4; st++) orc_conv_direct(spectra + L * st, L, s->G, L);
+        real ic = (spectra[0] > spectra[L - 1]) ? spectra[0] : spectra[L - 1];
+        conv_direct_ic_r(spectra, L, s->G, L, ic);
+        for (int st = 1; st < 4; st++) conv_direct_r(spectra + L * st, L, s->G, L);
     }
     if (s->use_slight) {                                            /* :405-412 */
         for (int i = 0; i < 4 * L; i++) {
             ws->spectra_slight[i] = spectra[i];
-            spectra[i] = spectra[i] * ALF + s->slight[i] * (1.0 - ALF);
+            spectra[i] = spectra[i] * ALF + (real)s->slight[i] * (1.0 - ALF);
         }
     }
 }
@@ -403,28 +482,29 @@ static void synth_ws(const orc_setup_t *s, const double *m, double *spectra,
  * the reference's "spectra" argument (the unconvolved synthesis) and
  * spec_slight its "spectra_slight" argument. */
 static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
-                  const double *spec_in, const double *spec_slight, double *D)
+                  const real *spec_in, const real *spec_slight, real *D)
 {
     const int L = s->nlambda, NT = s->nterms;
     const double LD = m[ORC_DOPP], A = m[ORC_AA];
     const double B1 = -((m[ORC_S1]) * s->ah), MC = m[ORC_MAC], ALF = m[ORC_ALFA];
     const double E0 = ws->E0;
-    double dfi[3][4][ORC_MAXL], dshi[3][4][ORC_MAXL];
+    real dfi[3][4][ORC_MAXL], dshi[3][4][ORC_MAXL];
     memset(dfi, 0, sizeof(dfi));
     memset(dshi, 0, sizeof(dshi));
 
     const double *wgt[3] = {s->z.wep, s->z.web, s->z.wer};
-    double auxCte = (-s->lambda0) / (ORC_VLIGHT * LD);              /* :544-546 */
-    const double rk = s->exact_rho_deriv ? 2.0 : 1.0;
+    real auxCte = (-s->lambda0) / (ORC_VLIGHT * LD);                /* :544-546 */
+    const real rk = s->exact_rho_deriv ? 2.0 : 1.0;
     for (int g = 0; g < 3; g++) {            /* funcionComponentFor :536-609 */
         for (int c = 0; c < ws->ncomp[g]; c++) {     /* last component wins */
-            const double *H = ws->H[g][c], *F = ws->F[g][c];
-            const double *uu = ws->uu[g][c];
-            double nux = ws->shift[g][c];
+            const real *H = ws->H[g][c], *F = ws->F[g][c];
+            const real *uu = ws->uu[g][c];
+            real nux = ws->shift[g][c];
+            const real w = (real)wgt[g][c];
             for (int j = 0; j < L; j++) {
-                double dH_u = ((4 * A * F[j]) - (2 * uu[j] * H[j])) * wgt[g][c];
-                double dF_u = (ORC_RR - A * H[j] - 2 * uu[j] * F[j]) * wgt[g][c];
-                double uun = -uu[j] / LD;
+                real dH_u = ((4 * A * F[j]) - (2 * uu[j] * H[j])) * w;
+                real dF_u = (ORC_RR - A * H[j] - 2 * uu[j] * F[j]) * w;
+                real uun = -uu[j] / LD;
                 dfi[g][0][j] = dH_u * (-nux);
                 dfi[g][1][j] = dH_u * auxCte;
                 dfi[g][2][j] = (dH_u * uun);
@@ -438,10 +518,10 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
     }
 
     /* derivatives of eta/rho w.r.t. parameters 0..6, :93-199 */
-    double (*d_ei)[ORC_MAXL] = ws->d_ei, (*d_eq)[ORC_MAXL] = ws->d_eq;
-    double (*d_eu)[ORC_MAXL] = ws->d_eu, (*d_ev)[ORC_MAXL] = ws->d_ev;
-    double (*d_rq)[ORC_MAXL] = ws->d_rq, (*d_ru)[ORC_MAXL] = ws->d_ru;
-    double (*d_rv)[ORC_MAXL] = ws->d_rv;
+    real (*d_ei)[ORC_MAXL] = ws->d_ei, (*d_eq)[ORC_MAXL] = ws->d_eq;
+    real (*d_eu)[ORC_MAXL] = ws->d_eu, (*d_ev)[ORC_MAXL] = ws->d_ev;
+    real (*d_rq)[ORC_MAXL] = ws->d_rq, (*d_ru)[ORC_MAXL] = ws->d_ru;
+    real (*d_rv)[ORC_MAXL] = ws->d_rv;
     memset(ws->d_ei, 0, sizeof(ws->d_ei)); memset(ws->d_eq, 0, sizeof(ws->d_eq));
     memset(ws->d_eu, 0, sizeof(ws->d_eu)); memset(ws->d_ev, 0, sizeof(ws->d_ev));
     memset(ws->d_rq, 0, sizeof(ws->d_rq)); memset(ws->d_ru, 0, sizeof(ws->d_ru));
@@ -452,30 +532,30 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
         d_rq[0][i] = ws->rhoqn[i] / E0; d_ru[0][i] = ws->rhoun[i] / E0;
         d_rv[0][i] = ws->rhovn[i] / E0;
     }
-    double sc = E0 * ws->sinis_cosa / 2;                            /* :118-127 */
-    double ss = E0 * ws->sinis_sina / 2;
-    double E0_2 = E0 / 2.0;
-    double sindi_cosa = ws->sindi * ws->cosa, sindi_sina = ws->sindi * ws->sina;
-    double cosdi_E0_2 = (E0_2) * ws->cosdi;
-    double cosi_2_E0 = E0 * ws->cosi / 2.0;
-    double cosis_2_E0_2 = ws->cosis_2 * E0_2;
-    double sinis_E0_2 = ws->sinis * E0_2;
+    real sc = E0 * ws->sinis_cosa / 2;                              /* :118-127 */
+    real ss = E0 * ws->sinis_sina / 2;
+    real E0_2 = E0 / 2.0;
+    real sindi_cosa = ws->sindi * ws->cosa, sindi_sina = ws->sindi * ws->sina;
+    real cosdi_E0_2 = (E0_2) * ws->cosdi;
+    real cosi_2_E0 = E0 * ws->cosi / 2.0;
+    real cosis_2_E0_2 = ws->cosis_2 * E0_2;
+    real sinis_E0_2 = ws->sinis * E0_2;
     for (int j = 1; j < 5; j++) {                                   /* :130-154 */
         for (int i = 0; i < L; i++) {
-            double dfisum = dfi[1][j - 1][i] + dfi[2][j - 1][i];
+            real dfisum = dfi[1][j - 1][i] + dfi[2][j - 1][i];
             d_ei[j][i] = (dfi[0][j - 1][i] * sinis_E0_2 + dfisum * cosis_2_E0_2);
-            double aux = dfi[0][j - 1][i] - dfisum / 2;
+            real aux = dfi[0][j - 1][i] - dfisum / 2;
             d_eq[j][i] = (aux) * sc;
             d_eu[j][i] = (aux) * ss;
             d_ev[j][i] = (dfi[2][j - 1][i] - dfi[1][j - 1][i]) * cosi_2_E0;
-            double auxr = dshi[0][j - 1][i] - (dshi[1][j - 1][i] + dshi[2][j - 1][i]) / 2.0;
+            real auxr = dshi[0][j - 1][i] - (dshi[1][j - 1][i] + dshi[2][j - 1][i]) / 2.0;
             d_rq[j][i] = (auxr) * sc;
             d_ru[j][i] = (auxr) * ss;
             d_rv[j][i] = ((dshi[2][j - 1][i] - dshi[1][j - 1][i])) * cosi_2_E0;
         }
     }
-    double cosi_cosdi = ws->cosi * ws->cosdi * E0_2;                 /* :157-180 */
-    double sindi_E0_2 = ws->sindi * E0_2;
+    real cosi_cosdi = ws->cosi * ws->cosdi * E0_2;                   /* :157-180 */
+    real sindi_E0_2 = ws->sindi * E0_2;
     for (int i = 0; i < L; i++) {
         d_ei[5][i] = ws->fi[0][i] * sindi_E0_2 + (ws->parcial1[i]) * cosi_cosdi;
         d_eq[5][i] = ws->parcial2[i] * sindi_cosa;
@@ -485,7 +565,7 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
         d_ru[5][i] = ws->parcial3[i] * sindi_sina;
         d_rv[5][i] = (ws->shi[2][i] - ws->shi[1][i]) * cosdi_E0_2;
     }
-    double sinis_cosda = ws->sinis * ws->cosda, sinis_sinda = ws->sinis * ws->sinda;
+    real sinis_cosda = ws->sinis * ws->cosda, sinis_sinda = ws->sinis * ws->sinda;
     for (int i = 0; i < L; i++) {                                   /* :184-199 */
         d_eq[6][i] = ws->parcial2[i] * sinis_cosda;
         d_eu[6][i] = ws->parcial2[i] * sinis_sinda;
@@ -495,35 +575,35 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
 
     /* chain rule through the Unno-Rachkovsky solution, :207-277 */
     for (int i = 0; i < L; i++) {
-        double ei = ws->etai[i], eq = ws->etaq[i], eu = ws->etau[i], ev = ws->etav[i];
-        double rq = ws->rhoq[i], ru = ws->rhou[i], rv = ws->rhov[i];
-        double ei2 = ws->etai_2[i], gp1 = ws->gp1[i], gp2 = ws->gp2[i], gp3 = ws->gp3[i];
-        double dt = ws->dt[i];
-        double dtaux = (B1) / (dt * dt);
-        double etai_gp3 = ei * gp3;
-        double aux = 2 * ei;
-        double ext1 = aux * eq + ev * ru - eu * rv;
-        double ext2 = aux * eu + eq * rv - ev * rq;
-        double ext3 = aux * ev + eu * rq - eq * ru;
-        double ext4 = aux * gp1;
+        real ei = ws->etai[i], eq = ws->etaq[i], eu = ws->etau[i], ev = ws->etav[i];
+        real rq = ws->rhoq[i], ru = ws->rhou[i], rv = ws->rhov[i];
+        real ei2 = ws->etai_2[i], gp1 = ws->gp1[i], gp2 = ws->gp2[i], gp3 = ws->gp3[i];
+        real dt = ws->dt[i];
+        real dtaux = (B1) / (dt * dt);
+        real etai_gp3 = ei * gp3;
+        real aux = 2 * ei;
+        real ext1 = aux * eq + ev * ru - eu * rv;
+        real ext2 = aux * eu + eq * rv - ev * rq;
+        real ext3 = aux * ev + eu * rq - eq * ru;
+        real ext4 = aux * gp1;
         for (int il = 0; il < 7; il++) {
-            double dei = d_ei[il][i], deq = d_eq[il][i], deu = d_eu[il][i], dev = d_ev[il][i];
-            double drq = d_rq[il][i], dru = d_ru[il][i], drv = d_rv[il][i];
-            double dgp1 = 2.0 * (ei * dei - eq * deq - eu * deu - ev * dev + rq * drq +
-                                 ru * dru + rv * drv);
-            double dgp2 = rq * deq + eq * drq + ru * deu + eu * dru + rv * dev + ev * drv;
-            double d_dt = ext4 * dei + ei2 * dgp1 - 2 * gp2 * dgp2;
-            double dgp3 = 2.0 * (ei * dei + rq * drq + ru * dru + rv * drv);
+            real dei = d_ei[il][i], deq = d_eq[il][i], deu = d_eu[il][i], dev = d_ev[il][i];
+            real drq = d_rq[il][i], dru = d_ru[il][i], drv = d_rv[il][i];
+            real dgp1 = 2.0 * (ei * dei - eq * deq - eu * deu - ev * dev + rq * drq +
+                               ru * dru + rv * drv);
+            real dgp2 = rq * deq + eq * drq + ru * deu + eu * dru + rv * dev + ev * drv;
+            real d_dt = ext4 * dei + ei2 * dgp1 - 2 * gp2 * dgp2;
+            real dgp3 = 2.0 * (ei * dei + rq * drq + ru * dru + rv * drv);
             D[i + L * il] = -(((dei * gp3 + ei * dgp3) * dt - d_dt * etai_gp3) * (dtaux));
-            double dgp4 = dei * (ext1) + (ei2) * deq + ei * (ru * dev + ev * dru - rv * deu - eu * drv);
+            real dgp4 = dei * (ext1) + (ei2) * deq + ei * (ru * dev + ev * dru - rv * deu - eu * drv);
             D[i + L * il + L * NT] = ((dgp4 + drq * gp2 + rq * dgp2) * dt - d_dt * (ws->g4r[i])) * (dtaux);
-            double dgp5 = dei * (ext2) + (ei2) * deu + ei * (rv * deq + eq * drv - rq * dev - ev * drq);
+            real dgp5 = dei * (ext2) + (ei2) * deu + ei * (rv * deq + eq * drv - rq * dev - ev * drq);
             D[i + L * il + L * NT * 2] = ((dgp5 + dru * gp2 + ru * dgp2) * dt - d_dt * (ws->g5r[i])) * (dtaux);
-            double dgp6 = dei * (ext3) + (ei2) * dev + ei * (rq * deu + eu * drq - ru * deq - eq * dru);
+            real dgp6 = dei * (ext3) + (ei2) * dev + ei * (rq * deu + eu * drq - ru * deq - eq * dru);
             D[i + L * il + L * NT * 3] = ((dgp6 + drv * gp2 + rv * dgp2) * dt - d_dt * (ws->g6r[i])) * (dtaux);
         }
         /* S1 (:281-298) uses dti scaled in place by -ah */
-        double dtin = -(ws->dti[i] * s->ah);
+        real dtin = -(ws->dti[i] * s->ah);
         D[i + L * 8] = -dtin * etai_gp3;
         D[i + L * 8 + L * NT] = dtin * (ws->g4r[i]);
         D[i + L * 8 + L * NT * 2] = dtin * (ws->g5r[i]);
@@ -539,6 +619,15 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
     }
 
     int mac_applied = 0;
+    /* Without macroturbulence (v_mac <= 1e-4, :317) the reference leaves its
+     * v_mac row as the last call that had one computed it -- the previous
+     * model, or another pixel -- and convolves it with the PSF again
+     * (:471-495, every row but S0's).  Here it is zero, the derivative of
+     * the spectra with respect to v_mac at v_mac = 0 (the kernel's width
+     * enters at second order); the GPU kernels do the same. */
+    if (NT >= 10 && !(MC > 0.0001))
+        for (int st = 0; st < 4; st++)
+            for (int i = 0; i < L; i++) D[9 * L + L * NT * st + i] = 0;
     if (MC > 0.0001) {                                              /* :317-438 */
         mac_applied = 1;
         double gmac[ORC_MAXL], gder[ORC_MAXL];
@@ -557,42 +646,42 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
             orc_conv_direct(gmac, L, s->G, L);
         }
         for (int st = 0; st < 4; st++)       /* run twice in the reference */
-            orc_conv_circular(spec_in + L * st, gder, L, D + 9 * L + L * NT * st);
+            conv_circular_r(spec_in + L * st, gder, L, D + 9 * L + L * NT * st);
         for (int st = 0; st < 4; st++)
             for (int p = 0; p < 9; p++)
                 if (p != 7)
-                    orc_conv_circular(D + L * p + L * NT * st, gmac, L,
-                                      D + L * p + L * NT * st);
+                    conv_circular_r(D + L * p + L * NT * st, gmac, L,
+                                    D + L * p + L * NT * st);
     }
     if (!mac_applied && s->use_psf && s->use_fft) {                 /* :441-470 */
         for (int st = 0; st < 4; st++)
             for (int p = 0; p < NT; p++)
                 if (p != 7)
-                    orc_conv_circular(D + L * p + L * NT * st, s->G, L,
-                                      D + L * p + L * NT * st);
+                    conv_circular_r(D + L * p + L * NT * st, s->G, L,
+                                    D + L * p + L * NT * st);
     } else if (!mac_applied && s->use_psf) {                        /* :471-495 */
         for (int p = 0; p < NT; p++) {
             if (p == 7) continue;
-            double *row = D + L * p;
-            double ic = (row[0] > row[L - 1]) ? row[0] : row[L - 1];
-            orc_conv_direct_ic(row, L, s->G, L, ic);
+            real *row = D + L * p;
+            real ic = (row[0] > row[L - 1]) ? row[0] : row[L - 1];
+            conv_direct_ic_r(row, L, s->G, L, ic);
         }
         for (int st = 1; st < 4; st++)
             for (int p = 0; p < NT; p++)
-                if (p != 7) orc_conv_direct(D + L * p + L * NT * st, L, s->G, L);
+                if (p != 7) conv_direct_r(D + L * p + L * NT * st, L, s->G, L);
     }
     if (s->use_slight) {                                            /* :499-525 */
         for (int st = 0; st < 4; st++) {
             for (int p = 0; p < NT; p++) {
                 for (int i = 0; i < L; i++) {
-                    double *d = D + L * p + L * NT * st + i;
+                    real *d = D + L * p + L * NT * st + i;
                     *d = *d * ALF;
                     int ff_row = -1;
                     if (NT == 11) ff_row = 10;
                     else if (s->fix[9]) ff_row = 10;
                     else ff_row = 9;
                     if (p == ff_row)
-                        *d = spec_slight[L * st + i] - s->slight[L * st + i];
+                        *d = spec_slight[L * st + i] - (real)s->slight[L * st + i];
                 }
             }
         }
@@ -602,9 +691,17 @@ static void rf_ws(const orc_setup_t *s, const double *m, orc_ws_t *ws,
 void orc_synth(const orc_setup_t *s, const double model[ORC_NPAR],
                double *spectra, double *rf)
 {
+    const size_t L = s->nlambda, NT = s->nterms;
     orc_ws_t *ws = (orc_ws_t *)malloc(sizeof(orc_ws_t));
-    synth_ws(s, model, spectra, ws);
-    if (rf) rf_ws(s, model, ws, ws->spectra_mac, ws->spectra_slight, rf);  /* milos.c:601-602 */
+    real *sp = (real *)malloc(sizeof(real) * 4 * L);
+    real *d = rf ? (real *)calloc(4 * NT * L, sizeof(real)) : NULL;
+    synth_ws(s, model, sp, ws);
+    if (rf) rf_ws(s, model, ws, ws->spectra_mac, ws->spectra_slight, d);  /* milos.c:601-602 */
+    for (size_t i = 0; i < 4 * L; i++) spectra[i] = sp[i];
+    if (rf)
+        for (size_t i = 0; i < 4 * NT * L; i++) rf[i] = d[i];
+    free(d);
+    free(sp);
     free(ws);
 }
 
@@ -612,23 +709,23 @@ void orc_synth(const orc_setup_t *s, const double model[ORC_NPAR],
 /* Normal equations, chi2, solve, update                                     */
 /* ------------------------------------------------------------------------ */
 
-/* covarm2, lib.c:71-116, with the per-Stokes sums of lib.c:237-268. */
-void orc_covarm(const orc_setup_t *s, const double *sig, const float *obs,
-                const double *spectra, const double *rf, double *beta,
-                double *alpha)
+/* covarm2, lib.c:71-116, with the per-Stokes sums of lib.c:237-268: REAL
+ * sums (float in the reference's default build). */
+static void covarm_r(const orc_setup_t *s, const real *sig, const float *obs,
+                     const real *spectra, const real *rf, real *beta, real *alpha)
 {
     const int L = s->nlambda, NT = s->nterms;
-    double AP[4][ORC_NPAR * ORC_NPAR], BT[4][ORC_NPAR];
+    real AP[4][ORC_NPAR * ORC_NPAR], BT[4][ORC_NPAR];
     for (int st = 0; st < 4; st++) {
-        double opa[ORC_MAXL];
-        double w = s->weight[st];
+        real opa[ORC_MAXL];
+        real w = s->weight[st];
         for (int i = 0; i < L; i++) opa[i] = w * (spectra[i + L * st] - obs[i + L * st]);
-        const double *d = rf + st * L * NT;
+        const real *d = rf + st * L * NT;
         for (int i = 0; i < NT; i++) {
             for (int h = 0; h < NT; h++) {
-                double sum = 0, sum2 = 0;
+                real sum = 0, sum2 = 0;
                 for (int k = 0; k < L; k++) {
-                    double dAux = d[h * L + k];
+                    real dAux = d[h * L + k];
                     sum += (d[i * L + k] * dAux) * (w / sig[L * st + k]);
                     if (i == 0) sum2 += ((opa[k] * dAux)) / sig[L * st + k];
                 }
@@ -638,32 +735,56 @@ void orc_covarm(const orc_setup_t *s, const double *sig, const float *obs,
         }
     }
     for (int i = 0; i < NT; i++) {                                  /* lib.c:237-248 */
-        double sum = 0;
+        real sum = 0;
         for (int st = 0; st < 4; st++) sum += BT[st][i];
         beta[i] = sum;
     }
     for (int i = 0; i < NT * NT; i++) {                             /* lib.c:257-268 */
-        double sum = 0;
+        real sum = 0;
         for (int st = 0; st < 4; st++) sum += AP[st][i];
         alpha[i] = sum;
     }
 }
 
-/* fchisqr, lib.c:126-152. */
+/* fchisqr, lib.c:126-152 (REAL sums). */
+static real chisqr_r(const orc_setup_t *s, const real *spectra, const float *obs,
+                     const real *sig, real nfree)
+{
+    const int L = s->nlambda;
+    real o[4] = {0, 0, 0, 0};
+    for (int i = 0; i < L; i++) {
+        for (int st = 0; st < 4; st++) {
+            real d = spectra[i + L * st] - obs[i + L * st];
+            o[st] += (((d * d) * (real)s->weight[st]) / (sig[i + L * st]));
+        }
+    }
+    real tot = 0;
+    tot += o[0] + o[1] + o[2] + o[3];
+    return tot / nfree;
+}
+
+void orc_covarm(const orc_setup_t *s, const double *sig, const float *obs,
+                const double *spectra, const double *rf, double *beta,
+                double *alpha)
+{
+    const int L = s->nlambda, NT = s->nterms;
+    real sg[4 * ORC_MAXL], sp[4 * ORC_MAXL], b[ORC_NPAR], a[ORC_NPAR * ORC_NPAR];
+    real *d = (real *)malloc(sizeof(real) * 4 * (size_t)NT * (size_t)L);
+    for (int i = 0; i < 4 * L; i++) { sg[i] = sig[i]; sp[i] = spectra[i]; }
+    for (int i = 0; i < 4 * NT * L; i++) d[i] = rf[i];
+    covarm_r(s, sg, obs, sp, d, b, a);
+    for (int i = 0; i < NT; i++) beta[i] = b[i];
+    for (int i = 0; i < NT * NT; i++) alpha[i] = a[i];
+    free(d);
+}
+
 double orc_chisqr(const orc_setup_t *s, const double *spectra, const float *obs,
                   const double *sig, double nfree)
 {
     const int L = s->nlambda;
-    double o[4] = {0, 0, 0, 0};
-    for (int i = 0; i < L; i++) {
-        for (int st = 0; st < 4; st++) {
-            double d = spectra[i + L * st] - obs[i + L * st];
-            o[st] += (((d * d) * s->weight[st]) / (sig[i + L * st]));
-        }
-    }
-    double tot = 0;
-    tot += o[0] + o[1] + o[2] + o[3];
-    return tot / nfree;
+    real sg[4 * ORC_MAXL], sp[4 * ORC_MAXL];
+    for (int i = 0; i < 4 * L; i++) { sg[i] = sig[i]; sp[i] = spectra[i]; }
+    return chisqr_r(s, sp, obs, sg, nfree);
 }
 
 /* Symmetric eigen-decomposition by cyclic Jacobi rotations (own method; the
@@ -797,7 +918,7 @@ void orc_check(double m[ORC_NPAR])
 }
 
 /* FijaACeroDerivadasNoNecesarias, milosUtils.c:199-227. */
-void orc_fija_cero(const orc_setup_t *s, double *rf)
+static void fija_cero_r(const orc_setup_t *s, real *rf)
 {
     const int L = s->nlambda, NT = s->nterms;
     for (int p = 0; p < NT; p++) {
@@ -809,6 +930,16 @@ void orc_fija_cero(const orc_setup_t *s, double *rf)
         for (int st = 0; st < 4; st++)
             for (int i = 0; i < L; i++) rf[i + L * p + st * L * NT] = 0;
     }
+}
+
+void orc_fija_cero(const orc_setup_t *s, double *rf)
+{
+    const size_t n = 4 * (size_t)s->nterms * (size_t)s->nlambda;
+    real *d = (real *)malloc(sizeof(real) * n);
+    for (size_t i = 0; i < n; i++) d[i] = rf[i];
+    fija_cero_r(s, d);
+    for (size_t i = 0; i < n; i++) rf[i] = d[i];
+    free(d);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -906,14 +1037,100 @@ void orc_classical(const orc_setup_t *s, const float *spectro, double model[ORC_
     orc_classical_ex(s, spectro, model, 1);
 }
 
-/* lm_mils, milosUtils.c:466-635. */
+/* Test builds only (-DORC_PERTURB=eps): every synthesised sample and
+ * response value multiplied by 1 + eps * u, u uniform in [-1, 1) from a
+ * fixed-seed generator -- an implementation that differs from this one by
+ * rounding-level amounts everywhere.  The spread of the inversions under it
+ * measures how far such differences alone carry the LM paths. */
+#ifdef ORC_PERTURB
+static unsigned long long perturb_state = 0x9E3779B97F4A7C15ull;
+static void perturb(real *x, size_t n)
+{
+    for (size_t i = 0; i < n; i++) {
+        perturb_state ^= perturb_state << 13;
+        perturb_state ^= perturb_state >> 7;
+        perturb_state ^= perturb_state << 17;
+        double u = (double)(perturb_state >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+        x[i] = x[i] * (1.0 + ORC_PERTURB * u);
+    }
+}
+#define PERTURB(x, n) perturb((x), (n))
+#else
+#define PERTURB(x, n) ((void)0)
+#endif
+
+/* The step of mil_svd (milosUtils.c:237-271).  The reference takes it from
+ * GSL's symmetric eigen-solver, which is absent here; the default stands in
+ * for it with orc_pinv_solve.  Built with -DORC_SOLVE=1 (test builds only),
+ * another method with the same result in exact arithmetic: LDL^T on the
+ * active rows, with the pseudo-inverse when the factorisation fails or the
+ * smallest eigenvalue may lie within 1000x of the 1e-12 cutoff (1/trace of
+ * the inverse below 1e-9).  The spread between the two measures how far the
+ * choice of a correct solver moves the inversions. */
+#ifndef ORC_SOLVE
+#define ORC_SOLVE 0
+#endif
+static void solve_step(int nt, const double *h, const double *beta, double *delta)
+{
+    if (ORC_SOLVE == 1) {
+        double F[ORC_NPAR][ORC_NPAR], d[ORC_NPAR], y[ORC_NPAR];
+        int act[ORC_NPAR], ok = 1;
+        for (int i = 0; i < nt; i++) act[i] = h[i * nt + i] != 0.0;
+        for (int i = 0; i < nt; i++)
+            for (int j = 0; j < nt; j++)
+                F[i][j] = (act[i] && act[j]) ? h[i * nt + j] : (i == j ? 1.0 : 0.0);
+        for (int j = 0; j < nt && ok; j++) {
+            double dj = F[j][j];
+            for (int k = 0; k < j; k++) dj -= F[j][k] * F[j][k] * d[k];
+            if (!(dj > 0.0)) { ok = 0; break; }
+            d[j] = dj;
+            for (int i = j + 1; i < nt; i++) {
+                double v = F[i][j];
+                for (int k = 0; k < j; k++) v -= F[i][k] * F[j][k] * d[k];
+                F[i][j] = v / dj;
+            }
+        }
+        if (ok) {
+            /* trace of the inverse: sum over columns of L^-1 D^-1 L^-T */
+            double Li[ORC_NPAR][ORC_NPAR], tr = 0;
+            for (int c = 0; c < nt; c++) {
+                for (int i = 0; i < nt; i++) {
+                    double v = (i == c) ? 1.0 : 0.0;
+                    for (int k = c; k < i; k++) v -= F[i][k] * Li[k][c];
+                    Li[i][c] = i < c ? 0.0 : v;
+                }
+            }
+            for (int i = 0; i < nt; i++)
+                for (int k = 0; k <= i; k++)
+                    if (act[i]) tr += Li[i][k] * Li[i][k] / d[i];
+            if (tr < 1e9) {
+                for (int i = 0; i < nt; i++) {
+                    double v = act[i] ? beta[i] : 0.0;
+                    for (int k = 0; k < i; k++) v -= F[i][k] * y[k];
+                    y[i] = v;
+                }
+                for (int i = nt - 1; i >= 0; i--) {
+                    double v = y[i] / d[i];
+                    for (int k = i + 1; k < nt; k++) v -= F[k][i] * delta[k];
+                    delta[i] = act[i] ? v : 0.0;
+                }
+                return;
+            }
+        }
+    }
+    orc_pinv_solve(nt, h, beta, delta);
+}
+
+/* lm_mils, milosUtils.c:466-635.  REAL: damping, chi2, factor, alpha,
+ * beta, sigma, spectra, response functions; PRECISION: covar, delta, the
+ * model and toplim (:475-533). */
 int orc_lm(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],
            double *fit, float *chi2, int *iters, int *accepted)
 {
     const int L = s->nlambda, NT = s->nterms;
-    const double better = 5.0, worst = 10.0;                        /* :475-477 */
-    double factor = 1.0;
-    double sig[4 * ORC_MAXL];
+    const real better = 5.0, worst = 10.0;                          /* :475-477 */
+    real factor = 1.0;
+    real sig[4 * ORC_MAXL];
     int n_ghosts = 0;
     for (int i = 0; i < 4 * L; i++) {                               /* :484-492 */
         if (spectro[i] <= -1) { sig[i] = 1000000000000000000000.0; n_ghosts++; }
@@ -925,20 +1142,23 @@ int orc_lm(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],
     if (nfree == 0) return -1;                                      /* :509-512 */
 
     orc_ws_t *ws = (orc_ws_t *)malloc(sizeof(orc_ws_t));
-    double *spectra = (double *)malloc(sizeof(double) * 4 * (size_t)L);
-    double *rf = (double *)malloc(sizeof(double) * 4 * (size_t)NT * (size_t)L);
-    double beta[ORC_NPAR], alpha[ORC_NPAR * ORC_NPAR], covar[ORC_NPAR * ORC_NPAR];
+    real *spectra = (real *)malloc(sizeof(real) * 4 * (size_t)L);
+    real *rf = (real *)calloc(4 * (size_t)NT * (size_t)L, sizeof(real));
+    real beta[ORC_NPAR], alpha[ORC_NPAR * ORC_NPAR];
+    double covar[ORC_NPAR * ORC_NPAR], betad[ORC_NPAR];
     double delta[ORC_NPAR];
-    double flambda = s->ilambda;
+    real flambda = s->ilambda;
     int it = 0, nacc = 0, clanda = 0;
 
     synth_ws(s, model, spectra, ws);                                /* :536-539 */
+    PERTURB(spectra, 4 * (size_t)L);
     rf_ws(s, model, ws, ws->spectra_mac, ws->spectra_slight, rf);
-    orc_fija_cero(s, rf);
-    orc_covarm(s, sig, spectro, spectra, rf, beta, alpha);
+    PERTURB(rf, 4 * (size_t)NT * L);
+    fija_cero_r(s, rf);
+    covarm_r(s, sig, spectro, spectra, rf, beta, alpha);
     for (int i = 0; i < NT * NT; i++) covar[i] = alpha[i];
-    double ochisqr = orc_chisqr(s, spectra, spectro, sig, nfree);   /* :546-547 */
-    double chisqr0 = ochisqr;
+    real ochisqr = chisqr_r(s, spectra, spectro, sig, nfree);       /* :546-547 */
+    real chisqr0 = ochisqr;
     double trial[ORC_NPAR];
     memcpy(trial, model, sizeof(trial));                            /* :549 */
 
@@ -947,19 +1167,22 @@ int orc_lm(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],
             int ind = i * (NT + 1);
             covar[ind] = alpha[ind] * (1.0 + flambda);
         }
-        orc_pinv_solve(NT, covar, beta, delta);                      /* :565 */
+        for (int i = 0; i < NT; i++) betad[i] = beta[i];
+        solve_step(NT, covar, betad, delta);                         /* :565 */
         orc_aplica_delta(s, model, delta, trial);                    /* :566 */
         orc_check(trial);                                            /* :568 */
         synth_ws(s, trial, spectra, ws);                             /* :569 */
-        double chisqr = orc_chisqr(s, spectra, spectro, sig, nfree);
-        if ((fabs((ochisqr - chisqr) * 100 / chisqr) < s->toplim) || (chisqr < 0.0001))
+        PERTURB(spectra, 4 * (size_t)L);
+        real chisqr = chisqr_r(s, spectra, spectro, sig, nfree);
+        if ((r_fabs((ochisqr - chisqr) * 100 / chisqr) < s->toplim) || (chisqr < 0.0001))
             clanda = 1;                                              /* :579-580 */
         if (chisqr - ochisqr < 0.) {                                 /* :581-606 */
             flambda = flambda / (better * factor);
             memcpy(model, trial, sizeof(trial));
             rf_ws(s, model, ws, ws->spectra_mac, spectra, rf);      /* args as :587 */
-            orc_fija_cero(s, rf);
-            orc_covarm(s, sig, spectro, spectra, rf, beta, alpha);
+            PERTURB(rf, 4 * (size_t)NT * L);
+            fija_cero_r(s, rf);
+            covarm_r(s, sig, spectro, spectra, rf, beta, alpha);
             for (int i = 0; i < NT * NT; i++) covar[i] = alpha[i];
             ochisqr = chisqr;
             nacc++;
@@ -970,13 +1193,14 @@ int orc_lm(const orc_setup_t *s, const float *spectro, double model[ORC_NPAR],
         if ((flambda > 1e+7) || (flambda < 1e-25)) clanda = 1;       /* :617-618 */
         it++;
         if (s->logclambda)                                           /* :621 */
-            factor = (double)(log10f((float)chisqr) / log10f((float)chisqr0));
+            factor = (real)(log10f((float)chisqr) / log10f((float)chisqr0));
     } while (it < s->miter && !clanda);
 
     if (chi2) *chi2 = (float)ochisqr;                                /* :625 */
     if (iters) *iters = it;
     if (accepted) *accepted = nacc;
-    if (fit) memcpy(fit, spectra, sizeof(double) * 4 * (size_t)L);
+    if (fit)
+        for (int i = 0; i < 4 * L; i++) fit[i] = spectra[i];
     free(rf); free(spectra); free(ws);
     return 1;
 }

@@ -48,43 +48,68 @@ _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 
+# the builds of the restatement (oracle/Makefile): the FP64 checker, the
+# same at the reference's -O3 -march=native flags (FMA contraction), and
+# REAL = float, the reference's default build
+BUILDS = ("liboracle.so", "liboracle_o3.so", "liboracle_f32.so")
+_loaded = {}
+
+
 def lib():
+    """The oracle build in use: liboracle.so, or the one ORACLE_LIB names,
+    or the one use() selected."""
     global _lib
     if _lib is None:
-        # ORACLE_LIB=liboracle_o3.so selects the -O3 -march=native build
-        path = os.path.join(HERE, os.environ.get("ORACLE_LIB", "liboracle.so"))
-        if not os.path.exists(path):
-            raise RuntimeError("oracle not built: run make -C oracle")
-        L = C.CDLL(path)
-        L.orc_zeeman.argtypes = [C.c_double] * 6 + [C.POINTER(Zeeman)]
-        L.orc_gauss_psf.argtypes = [C.c_double, C.c_double, C.c_double,
-                                    C.c_double, C.c_int, _dp]
-        L.orc_nterms.argtypes = [_ip, C.c_double]
-        L.orc_fvoigt.argtypes = [C.c_double, _dp, C.c_int, _dp, _dp]
-        L.orc_gauss_mac.argtypes = [C.c_double, _dp, C.c_int, C.c_double,
-                                    C.c_int, _dp]
-        L.orc_conv_direct.argtypes = [_dp, C.c_int, _dp, C.c_int]
-        L.orc_conv_direct_ic.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_double]
-        L.orc_conv_circular.argtypes = [_dp, _dp, C.c_int, _dp]
-        L.orc_synth.argtypes = [C.POINTER(Setup), _dp, _dp, C.c_void_p]
-        L.orc_covarm.argtypes = [C.POINTER(Setup), _dp, _fp, _dp, _dp, _dp, _dp]
-        L.orc_chisqr.argtypes = [C.POINTER(Setup), _dp, _fp, _dp, C.c_double]
-        L.orc_chisqr.restype = C.c_double
-        L.orc_pinv_solve.argtypes = [C.c_int, _dp, _dp, _dp]
-        L.orc_eigen_sym.argtypes = [C.c_int, _dp, _dp, _dp]
-        L.orc_aplica_delta.argtypes = [C.POINTER(Setup), _dp, _dp, _dp]
-        L.orc_check.argtypes = [_dp]
-        L.orc_fija_cero.argtypes = [C.POINTER(Setup), _dp]
-        L.orc_classical.argtypes = [C.POINTER(Setup), _fp, _dp]
-        L.orc_classical_ex.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_int]
-        L.orc_lm.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_void_p,
-                             C.POINTER(C.c_float), C.POINTER(C.c_int),
-                             C.POINTER(C.c_int)]
-        L.orc_lm.restype = C.c_int
-        L.orc_invert_range.argtypes = [C.POINTER(Setup), _fp, C.c_int, C.c_int,
-                                       _dp, _dp, _fp, _ip, _ip, C.c_void_p]
-        _lib = L
+        _lib = _load(os.environ.get("ORACLE_LIB", "liboracle.so"))
     return _lib
+
+
+def use(name):
+    """Selects the oracle build `name` (one of BUILDS) for the calls that
+    follow; returns the name of the one in use before."""
+    global _lib
+    prev = getattr(_lib, "_oracle_name", None) or os.environ.get("ORACLE_LIB", "liboracle.so")
+    _lib = _load(name)
+    return prev
+
+
+def _load(name):
+    if name in _loaded:
+        return _loaded[name]
+    path = os.path.join(HERE, name)
+    if not os.path.exists(path):
+        raise RuntimeError("oracle not built: run make -C oracle")
+    L = C.CDLL(path)
+    L._oracle_name = name
+    L.orc_zeeman.argtypes = [C.c_double] * 6 + [C.POINTER(Zeeman)]
+    L.orc_gauss_psf.argtypes = [C.c_double, C.c_double, C.c_double,
+                                C.c_double, C.c_int, _dp]
+    L.orc_nterms.argtypes = [_ip, C.c_double]
+    L.orc_fvoigt.argtypes = [C.c_double, _dp, C.c_int, _dp, _dp]
+    L.orc_gauss_mac.argtypes = [C.c_double, _dp, C.c_int, C.c_double,
+                                C.c_int, _dp]
+    L.orc_conv_direct.argtypes = [_dp, C.c_int, _dp, C.c_int]
+    L.orc_conv_direct_ic.argtypes = [_dp, C.c_int, _dp, C.c_int, C.c_double]
+    L.orc_conv_circular.argtypes = [_dp, _dp, C.c_int, _dp]
+    L.orc_synth.argtypes = [C.POINTER(Setup), _dp, _dp, C.c_void_p]
+    L.orc_covarm.argtypes = [C.POINTER(Setup), _dp, _fp, _dp, _dp, _dp, _dp]
+    L.orc_chisqr.argtypes = [C.POINTER(Setup), _dp, _fp, _dp, C.c_double]
+    L.orc_chisqr.restype = C.c_double
+    L.orc_pinv_solve.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_eigen_sym.argtypes = [C.c_int, _dp, _dp, _dp]
+    L.orc_aplica_delta.argtypes = [C.POINTER(Setup), _dp, _dp, _dp]
+    L.orc_check.argtypes = [_dp]
+    L.orc_fija_cero.argtypes = [C.POINTER(Setup), _dp]
+    L.orc_classical.argtypes = [C.POINTER(Setup), _fp, _dp]
+    L.orc_classical_ex.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_int]
+    L.orc_lm.argtypes = [C.POINTER(Setup), _fp, _dp, C.c_void_p,
+                         C.POINTER(C.c_float), C.POINTER(C.c_int),
+                         C.POINTER(C.c_int)]
+    L.orc_lm.restype = C.c_int
+    L.orc_invert_range.argtypes = [C.POINTER(Setup), _fp, C.c_int, C.c_int,
+                                   _dp, _dp, _fp, _ip, _ip, C.c_void_p]
+    _loaded[name] = L
+    return L
 
 
 def ref():

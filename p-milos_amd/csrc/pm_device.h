@@ -801,7 +801,13 @@ struct Classic {
 
 // estimacionesClasicas, milosUtils.c:287-452, on one observed profile (in
 // LDS when each lane of a pixel group runs it redundantly, in global memory
-// for the estimates-only pass), in the reference's summation order.  With
+// for the estimates-only pass), in the reference's summation order and
+// operand types, without contracting products into sums (the reference's
+// own build has no FMA, makefile:8).  I + V and I - V are float sums
+// (milosUtils.c:337,357): formed in double they differ by up to half a
+// float ulp per sample, which the LM's first, often ill-conditioned step
+// turns into visibly different paths (2% of the pixels of a 112-sample
+// cube with v_mac free end outside the FP64 tolerance).  With
 // init (forInitialUse = 1, the inversion's starting point) the 85-95 degree
 // inclination nudge applies to B and the pixel loop's NaN guards follow
 // (milos.c:976-983); without it (the estimates-only mode, milos.c:493) B_los
@@ -810,6 +816,7 @@ struct Classic {
 __device__ __noinline__ Classic classical(PmC c, const float *spec,
                                          bool init)
 {
+#pragma clang fp contract(off)
     const int n = c->L;
     const double l0 = c->lambda0;
     const float *sI = spec, *sQ = spec + n, *sU = spec + 2 * n, *sV = spec + 3 * n;
@@ -819,9 +826,11 @@ __device__ __noinline__ Classic classical(PmC c, const float *spec,
     for (int i = n - 1; i >= 0; i--) { if (sI[i] < 0) end--; else break; }
     double Ic = (end >= 1) ? (double)spec[end - 1] : 0.0;
     double x = 0, y = 0, xv = 0, yv = 0;
+    // I + V and I - V are float sums, as the reference forms them from its
+    // float spectro (milosUtils.c:337,357)
     for (int i = begin; i < end - 1; i++) {
         if (sI[i] > -1 && sV[i] > -1) {
-            double aux = (Ic - ((double)sI[i] + (double)sV[i]));
+            double aux = (Ic - (double)__fadd_rn(sI[i], sV[i]));
             double auxv = (Ic - (double)sI[i]);
             double dl = c->lam[i] - l0;
             x += (aux * dl);
@@ -834,7 +843,7 @@ __device__ __noinline__ Classic classical(PmC c, const float *spec,
     x = 0; y = 0;
     for (int i = begin; i < end - 1; i++) {
         if (sI[i] > -1 && sV[i] > -1) {
-            double aux = (Ic - ((double)sI[i] - (double)sV[i]));
+            double aux = (Ic - (double)__fsub_rn(sI[i], sV[i]));
             x += (aux * (c->lam[i] - l0));
             y += aux;
         }
@@ -900,8 +909,11 @@ __device__ __noinline__ Classic classical(PmC c, const float *spec,
 
 // The same estimates computed by a pixel group: each lane takes its own
 // samples (lambda = t + k*W) and the sums over wavelength are group sums
-// (tree order instead of the serial order), the range of the leading and
-// trailing negative I a group min/max.  Every lane returns the same result.
+// (tree order instead of the serial order: the sums' rounding differs from
+// the serial ones by a few units in the last place, which parts no more LM
+// paths than rebuilding the reference with other flags does), the range of
+// the leading and trailing negative I a group min/max.  I + V and I - V are
+// float sums as in the reference.  Every lane returns the same result.
 template <int W, int NLPL>
 __device__ __noinline__ Classic classical_group(PmC c, const float *spec, bool init, int t)
 {
@@ -926,13 +938,13 @@ __device__ __noinline__ Classic classical_group(PmC c, const float *spec, bool i
         const double dl = c->dlam[i];
         if (i >= begin && i < end - 1) {
             if (sI[i] > -1 && sV[i] > -1) {                       // :330-371
-                double aux = (Ic - ((double)sI[i] + (double)sV[i]));
+                double aux = (Ic - (double)__fadd_rn(sI[i], sV[i]));
                 double auxv = (Ic - (double)sI[i]);
                 x += (aux * dl);
                 xv += (auxv * dl);
                 y += aux;
                 yv += auxv;
-                double auxm = (Ic - ((double)sI[i] - (double)sV[i]));
+                double auxm = (Ic - (double)__fsub_rn(sI[i], sV[i]));
                 xm += (auxm * dl);
                 ym += auxm;
             }
@@ -992,4 +1004,3 @@ __device__ __noinline__ Classic classical_group(PmC c, const float *spec, bool i
     }
     return r;
 }
-

@@ -1,23 +1,18 @@
 """GPU parity: the HIP library (through its C ABI) against the CPU oracle on
 the same seeded inputs.  FP64 synthesis and response functions must agree
-within 1e-10 relative; inversions per parameter on >= 99.5% of pixels."""
+within 1e-10 relative; inversions within each case's stated tolerance
+(tests/cases.py) on every parameter on >= 99.5% of pixels."""
 import sys
 
 import numpy as np
 import pytest
 
+import cases
 import oracle as O
 import pmilos
-from fixtures import (FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models,
-                      oracle_cube)
+from fixtures import FE6173, FE6302, GUESS, TRIPLET, draw_models
 
 pytestmark = pytest.mark.gpu
-
-# per-parameter tolerance of the FP64 inversion against the FP64 oracle
-# (absolute, in the parameter's unit): eta0, B [G], vlos [km/s], dopp [A],
-# a, gamma [deg], phi [deg], S0, S1
-TOL64 = np.array([1e-3, 1e-2, 1e-6, 1e-8, 1e-6, 1e-4, 1e-4, 1e-6, 1e-6])
-
 
 def say(*a):
     print(*a, flush=True)
@@ -101,102 +96,66 @@ def test_synthesis_and_rf_psf_widths(fwhm, precision):
             assert np.abs(spec_g[i] - sp).max() < 1e-6 * np.abs(sp[0]).max(), i
             assert _err_rf(rf_g[i], rf) < 2e-3, i
     if precision == pmilos.PM_FP64:
-        truths = draw_models(256, seed=1301)
-        cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=1302)
-        # a PSF twice the benchmark's makes the fits less well conditioned:
-        # the wider tolerance of the other samplings, and a few weakly
-        # polarised pixels (3 of 256 with this seed) end on another branch of
-        # the azimuth when rounding-level differences part their LM paths
-        _compare_inversions(ctx.invert(cube), O.invert(s, cube, GUESS), "psf %g" % fwhm,
-                            tol=TOL64 if fwhm < 100 else TOL_OTHER[112],
-                            frac=0.995 if fwhm < 100 else 0.98)
+        name = "psf%d" % fwhm
+        say("case invert %s" % name)
+        c = cases.get(name)
+        cases.compare(c.context(pmilos).invert(c.cube), name, "fp64", say, cube=c.cube)
 
 
-def _compare_inversions(g, o, label, tol=TOL64, frac=0.995):
-    dm = np.abs(g["model"][:, :9] - o["model"][:, :9])
-    ok = dm <= tol
-    say("%s: per-parameter fraction within tolerance %s" % (label, np.round(ok.mean(axis=0), 4)))
-    say("%s: 99.5th percentile |d| %s" % (label, np.array2string(
-        np.nanpercentile(dm, 99.5, axis=0), precision=3)))
-    same_it = (g["iters"] == o["iters"]).mean()
-    say("%s: iterations identical on %.4f of pixels; mean iters gpu %.2f oracle %.2f; "
-        "mean |d iters| %.3f" % (label, same_it, g["iters"].mean(), o["iters"].mean(),
-                                 np.abs(g["iters"] - o["iters"]).mean()))
-    assert np.all(ok.mean(axis=0) >= frac), ok.mean(axis=0)
-    # chi2 agrees wherever the models agree to the tight FP64 tolerance
-    both = np.all(dm <= TOL64, axis=1)
-    np.testing.assert_allclose(g["chi2"][both], o["chi2"][both], rtol=1e-4)
+def _invert_case(name, **kw):
+    say("case invert %s" % name)
+    c = cases.get(name)
+    g = c.context(pmilos, **kw).invert(c.cube)
+    cases.compare(g, name, "fp64", say, cube=c.cube)
+    return c, g
 
 
 def test_invert_fp64_config1_field():
     """The CPU-runnable field: 64x64 pixels of seed-1 models, sigma 1e-3 Ic,
     PSF 60 mA, 30 lambda, 9 free, <= 50 iterations; the GPU and the oracle
     invert the same float32 cube."""
-    say("case invert L=30 npix=4096 seed=1")
-    s, ctx = _pair(30)
-    truths = draw_models(64 * 64, seed=1)
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=11)
-    o = O.invert(s, cube, GUESS)
-    g = ctx.invert(cube, want_fit=True)
-    _compare_inversions(g, o, "L30")
+    say("case invert l30_config1")
+    c = cases.get("l30_config1")
+    ctx = c.context(pmilos)
+    g = ctx.invert(c.cube, want_fit=True)
+    f = cases.compare(g, "l30_config1", "fp64", say, cube=c.cube)
+    # at the FP64 floor tolerance alone (every parameter to rounding-level
+    # agreement) on more than 99.5% of the pixels
+    ref = cases.reference("l30_config1")
+    tight = (np.abs(g["model"] - ref["model"]) <= cases.FLOOR).mean(axis=0)
+    say("l30_config1 within the floor tolerance %s" % np.round(tight, 4))
+    assert np.all(tight >= 0.995)
     assert ctx.fallback_count() >= 0
+    # chi2 agrees wherever the models agree at the floor
+    both = np.all(np.abs(g["model"] - ref["model"]) <= cases.FLOOR, axis=1)
+    np.testing.assert_allclose(g["chi2"][both], ref["chi2"][both], rtol=1e-4)
     # fits are the last trial synthesis of each pixel
-    o_fit = O.invert(s, cube[:16], GUESS, want_fit=True)["fit"]
-    same = g["iters"][:16] == o["iters"][:16]
-    np.testing.assert_allclose(g["fit"][:16][same], o_fit[same], rtol=1e-5, atol=1e-6)
+    s = c.setup()
+    o16 = O.invert(s, c.cube[:16], GUESS, want_fit=True)
+    same = g["iters"][:16] == o16["iters"]
+    np.testing.assert_allclose(g["fit"][:16][same], o16["fit"][same], rtol=1e-5, atol=1e-6)
 
 
-# With 6 wavelengths (15 degrees of freedom) many pixels run to the 50-trip
-# cap still moving; rounding-level differences (summation order, the LDL^T
-# vs eigen solve) then part the LM paths of a few percent of pixels, as the
-# reference's own different builds do (README.md:65).  Tolerances for these
-# samplings are stated separately (absolute, same parameter order).
-TOL_OTHER = {6: np.array([30.0, 60.0, 0.3, 5e-3, 0.1, 10.0, 20.0, 0.05, 0.05]),
-             112: np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 5e-3, 5e-3])}
-
-
-@pytest.mark.parametrize("L", [6, 112])
-def test_invert_fp64_other_samplings(L):
-    say("case invert L=%d npix=256" % L)
-    s, ctx = _pair(L)
-    truths = draw_models(256, seed=40 + L)
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=7)
-    o = O.invert(s, cube, GUESS)
-    g = ctx.invert(cube)
-    _compare_inversions(g, o, "L%d tight" % L, frac=0.0)
-    _compare_inversions(g, o, "L%d" % L, tol=TOL_OTHER[L])
+@pytest.mark.parametrize("name", ["l6", "l112"])
+def test_invert_fp64_other_samplings(name):
+    """6 samples (many pixels end at the 50-trip cap still moving) and 112
+    (Fe I 6302.5)."""
+    _invert_case(name)
 
 
 def test_invert_edge_cases():
-    """Ghost points, a pixel with nfree == 0 behaviour inputs, NaN spectra,
-    a fixed parameter and logclambda."""
-    say("case invert edge cases")
-    s, ctx = _pair(30, fix=[1, 1, 1, 1, 0, 1, 1, 1, 1, 0, 0], logclambda=0)
-    truths = draw_models(64, seed=5)
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=6)
-    cube[3, 1, 4] = -5.0                 # ghost point
-    cube[4, 0, :3] = -2.0                # leading ghost wavelengths in I
-    cube[5, 2, :] = -3.0                 # ghost U
-    cube[6] = np.nan                     # NaN pixel
-    o = O.invert(s, cube, GUESS)
-    g = ctx.invert(cube)
-    fin = np.isfinite(o["model"][:, 0])
-    assert np.array_equal(fin, np.isfinite(g["model"][:, 0]))
-    sub = lambda r: {k: v[fin] for k, v in r.items() if v is not None}
-    _compare_inversions(sub(g), sub(o), "edge", frac=0.98)
+    """Ghost points (a sample, leading I samples, a whole Stokes row), a NaN
+    pixel and a fixed damping."""
+    c, g = _invert_case("edge")
+    ref = cases.reference("edge")
+    assert np.array_equal(np.isfinite(ref["model"][:, 0]), np.isfinite(g["model"][:, 0]))
     assert np.all(g["model"][:, 4] == GUESS[4])          # damping fixed
 
 
 def test_invert_logclambda():
-    say("case invert logclambda=1")
-    s, ctx = _pair(30, logclambda=1)
-    truths = draw_models(256, seed=9)
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=10)
-    o = O.invert(s, cube, GUESS)
-    g = ctx.invert(cube)
-    # glibc log10f vs the correctly rounded value can move the damping factor
-    # by 1-2 ulp of float, so a few trajectories may part near convergence
-    _compare_inversions(g, o, "logc", tol=TOL64 * 1e3, frac=0.97)
+    """logclambda = 1.  glibc log10f vs the correctly rounded value can move
+    the damping factor by 1-2 ulp of float."""
+    _invert_case("logc")
 
 
 def test_nfree_zero_pixel():

@@ -7,11 +7,11 @@ whole field."""
 import numpy as np
 import pytest
 
+import cases
 import oracle as O
 import pmilos
 from fixtures import FE6173, GUESS, TRIPLET, draw_models
-from test_gpu_fp64 import TOL64, _compare_inversions, say
-from test_gpu_mixed import TOLMIX
+from test_gpu_fp64 import say
 
 pytestmark = pytest.mark.gpu
 
@@ -87,7 +87,13 @@ def test_pixels_across_the_field_match_the_oracle(field, fp64):
     s = O.make_setup(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam))
     o = O.invert(s, cube, GUESS)
     g = {k: v[idx] for k, v in fp64.items()}
-    _compare_inversions(g, o, "full field sample", tol=TOL64, frac=0.99)
+    # the tolerance of the 64 x 64 field of the same sampling, noise and
+    # model distribution (tests/cases.py, l30_config1)
+    tol = cases.tolerance("l30_config1", "fp64")
+    f = (np.abs(g["model"] - o["model"]) <= tol).mean(axis=0)
+    say("full field sample: fraction within tolerance %s; trips identical on %.4f"
+        % (np.round(f, 4), (g["iters"] == o["iters"]).mean()))
+    assert np.all(f >= cases.FRAC), f
 
 
 def test_bits_do_not_depend_on_order_or_workgroup(field, fp64):
@@ -113,14 +119,17 @@ def test_bits_do_not_depend_on_order_or_workgroup(field, fp64):
         assert np.array_equal(shuffled[k], fp64[k][perm]), k
 
 
-def test_mixed_form_within_tolerance_of_fp64_on_every_pixel(field, fp64):
+def test_mixed_form_within_tolerance_of_fp64_on_995_of_pixels(field, fp64):
+    """The mixed form against the FP64 form over all 1,048,576 pixels, at
+    the mixed tolerance of the same models and noise (tests/cases.py,
+    mix_s1e-3), on at least 99.5% of the pixels per parameter."""
     say("case full field mixed npix=%d" % NPIX)
     cm = pmilos.Context(FE6173, TRIPLET, field["lam"], psf=field["G"], guess=GUESS,
                         noise=1e-6, miter=50, precision=pmilos.PM_MIXED)
     mx = _run(field, cm, field["stokes"])
-    d = np.abs(mx["model"][:, :9] - fp64["model"][:, :9])
-    frac = (d <= TOLMIX).mean(axis=0)
+    d = np.abs(mx["model"] - fp64["model"])
+    frac = (d <= cases.tolerance("mix_s1e-3", "mixed")).mean(axis=0)
     say("full field mixed vs fp64: fraction within tolerance %s" % np.round(frac, 4))
     say("full field mixed: mean iters %.3f (fp64 %.3f), identical on %.4f"
         % (mx["iters"].mean(), fp64["iters"].mean(), (mx["iters"] == fp64["iters"]).mean()))
-    assert np.all(frac >= 0.99), frac
+    assert np.all(frac >= cases.FRAC), frac

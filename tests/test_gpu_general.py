@@ -8,6 +8,7 @@ import sys
 import numpy as np
 import pytest
 
+import cases
 import oracle as O
 import pmilos
 from fixtures import (FE6173, FE6302, GUESS, TRIPLET, add_noise, draw_models,
@@ -145,56 +146,31 @@ def test_synthesis_and_rf_general(case, L, psf):
         assert erm < 2e-3, (i, erm)
 
 
-# per-parameter tolerance of the general FP64 inversion against the oracle
-# (absolute, Init_Model order incl. v_mac [km/s] and the filling factor).
-# Wide grids with a free v_mac give the LM several nearly flat directions;
-# rounding-level differences (summation order, LDL^T vs eigen solve) part
-# a few paths, as with the 112-sample grid of the 9-parameter form.
-TOL_GEN = np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 5e-3, 5e-3, 5e-3, 5e-3])
-
-
-def _compare(g, o, label, tol, frac):
-    dm = np.abs(g["model"] - o["model"])
-    ok = dm <= tol
-    say("%s: fraction within tolerance %s" % (label, np.round(ok.mean(axis=0), 4)))
-    say("%s: 99th percentile |d| %s" % (label, np.array2string(
-        np.percentile(dm, 99, axis=0), precision=3)))
-    say("%s: iterations identical on %.4f; mean iters gpu %.2f oracle %.2f"
-        % (label, (g["iters"] == o["iters"]).mean(), g["iters"].mean(), o["iters"].mean()))
-    assert np.all(ok.mean(axis=0) >= frac), ok.mean(axis=0)
-
-
-@pytest.mark.parametrize("case,L,psf", [("mac_nt10", 112, True), ("mac_nt11_sl", 49, False),
-                                        ("ff_nt10_sl", 30, True), ("nt9_sl", 30, True)])
-def test_invert_general_fp64(case, L, psf):
-    """Inversions of a seeded cube (sigma 1e-3 Ic) with v_mac ~ U(0, 2) km/s
-    and, with stray light, alpha ~ U(0.6, 1): FP64 against the oracle."""
-    n = 256
-    say("case general invert %s L=%d psf=%s npix=%d" % (case, L, psf, n))
-    s, ctx, guess, sl = _pair(case, L, psf)
-    truths = _models(n, 3000 + L, case)
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=31)
-    o = O.invert(s, cube, guess)
-    g = ctx.invert(cube)
-    _compare(g, o, "%s L%d" % (case, L), TOL_GEN, 0.98)
+@pytest.mark.parametrize("name", ["gen_mac_nt10_l112", "gen_mac_nt11_sl_l49",
+                                  "gen_ff_nt10_sl_l30", "gen_nt9_sl_l30"])
+def test_invert_general_fp64(name):
+    """Inversions of seeded cubes (sigma 1e-3 Ic) with v_mac ~ U(0, 2) km/s
+    and, with stray light, alpha ~ U(0.6, 1): FP64 against the oracle within
+    the case's stated tolerance on >= 99.5% of pixels per parameter."""
+    say("case general invert %s" % name)
+    c = cases.get(name)
+    g = c.context(pmilos).invert(c.cube)
+    cases.compare(g, name, "fp64", say, cube=c.cube)
     # parameters that are not free keep the guess
+    fix = c.ctx_kw["fix"]
     for i in range(9, 11):
-        if not CASES[case][0][i]:
-            assert np.all(g["model"][:, i] == guess[i])
+        if not fix[i]:
+            assert np.all(g["model"][:, i] == c.guess[i])
 
 
 def test_invert_general_mixed():
     """The mixed form of the 112-sample v_mac inversion against the FP64
-    oracle, with the mixed form's own tolerances."""
-    n = 256
-    say("case general invert mixed mac_nt10 L=112 npix=%d" % n)
-    s, ctx, guess, sl = _pair("mac_nt10", 112, True, precision=pmilos.PM_MIXED)
-    truths = _models(n, 3112, "mac_nt10")
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=31)
-    o = O.invert(s, cube, guess)
-    g = ctx.invert(cube)
-    tol = np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 1e-2, 1e-2, 1e-2, 1e-2])
-    _compare(g, o, "mixed mac_nt10 L112", tol, 0.97)
+    oracle, within its stated mixed tolerance."""
+    name = "gen_mac_nt10_l112"
+    say("case general invert mixed %s" % name)
+    c = cases.get(name)
+    g = c.context(pmilos, precision=pmilos.PM_MIXED).invert(c.cube)
+    cases.compare(g, name, "mixed", say, cube=c.cube)
 
 
 def test_invert_per_pixel_stray_light():
@@ -202,32 +178,16 @@ def test_invert_per_pixel_stray_light():
     milos.c:986-996): each pixel against the oracle set up with its own
     profile, and a cube repeating the context's profile gives exactly the
     context-profile inversion."""
-    n, L = 48, 30
-    say("case per-pixel stray light L=%d npix=%d" % (L, n))
-    s0, ctx, guess, sl = _pair("ff_nt10_sl", L, True)
-    truths = _models(n, 4000, "ff_nt10_sl")
-    lam = O.grid(*GRIDS[L])
-    G = O.gauss_psf(60.0, lam)
-    rng = np.random.default_rng(41)
-    scl = (sl[None] * rng.uniform(0.8, 1.2, (n, 1, 1))).astype(np.float32)
-    fix = CASES["ff_nt10_sl"][0]
-    cube = np.zeros((n, 4, L), dtype=np.float32)
-    outs = []
-    for i in range(n):
-        si = O.make_setup(FE6173, TRIPLET, lam, psf=G, fix=fix, slight=scl[i].astype(np.float64))
-        cube[i] = add_noise(O.synth(si, truths[i])[None], truths[i:i + 1], 1e-3, seed=50 + i)[0]
-        outs.append((si, cube[i:i + 1]))
-    g = ctx.invert(cube, slight=scl)
-    o = {k: [] for k in ("model", "chi2", "iters")}
-    for si, c in outs:
-        r = O.invert(si, c, guess)
-        for k in o:
-            o[k].append(r[k][0])
-    o = {k: np.array(v) for k, v in o.items()}
-    _compare(g, o, "per-pixel slight", TOL_GEN, 0.95)
-    rep = np.repeat(sl[None], n, axis=0)
-    a = ctx.invert(cube, slight=rep)
-    b = ctx.invert(cube)
+    name = "per_pixel_slight"
+    say("case per-pixel stray light %s" % name)
+    c = cases.get(name)
+    ctx = c.context(pmilos)
+    g = ctx.invert(c.cube, slight=c.slight)
+    cases.compare(g, name, "fp64", say, cube=c.cube)
+    sl = c.ctx_kw["slight"]
+    rep = np.repeat(sl[None], len(c.cube), axis=0)
+    a = ctx.invert(c.cube[:256], slight=rep[:256])
+    b = ctx.invert(c.cube[:256])
     assert np.array_equal(a["model"], b["model"]) and np.array_equal(a["iters"], b["iters"])
 
 
@@ -268,13 +228,9 @@ def test_synthesis_and_rf_fft_mode(case, L):
 def test_invert_fft_mode(precision):
     """FFT-mode inversions of a seeded 30-sample cube (sigma 1e-3 Ic)
     against the oracle's FFT mode."""
-    n = 256
-    say("case fft invert nt9 L=30 precision=%d npix=%d" % (precision, n))
-    s, ctx, guess, sl = _pair("nt9", 30, True, precision=precision, use_fft=True)
-    truths = _models(n, 6030, "nt9")
-    cube = add_noise(oracle_cube(s, truths), truths, 1e-3, seed=37)
-    o = O.invert(s, cube, guess)
-    g = ctx.invert(cube)
-    tol = TOL_GEN if precision == pmilos.PM_FP64 else \
-        np.array([0.5, 5.0, 5e-3, 5e-4, 5e-3, 1.0, 2.0, 1e-2, 1e-2, 1e-2, 1e-2])
-    _compare(g, o, "fft nt9 L30 p%d" % precision, tol, 0.98 if precision == 0 else 0.97)
+    name = "fft_nt9_l30"
+    form = "fp64" if precision == pmilos.PM_FP64 else "mixed"
+    say("case fft invert %s %s" % (name, form))
+    c = cases.get(name)
+    g = c.context(pmilos, precision=precision).invert(c.cube)
+    cases.compare(g, name, form, say, cube=c.cube)

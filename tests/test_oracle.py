@@ -33,6 +33,45 @@ def test_golden_synthesis(golden_dir):
     assert np.all(np.abs(spec - g) <= 3e-7 + 1e-6 * np.abs(g))
 
 
+def test_float_build_prints_the_golden_synthesis_digit_for_digit(golden_dir):
+    """The REAL = float build of the restatement (liboracle_f32.so, the
+    reference's default build, makefile:21-25, without FMA contraction as
+    its icc line has it, makefile:8) prints every one of the 196 values of
+    test/synthesis_results.per exactly in the file's %e format; the FP64
+    build matches them only to the file's 7 digits."""
+    slight = read_fits_2d(os.path.join(golden_dir, "straylight.fits"))
+    lam = O.grid(FE6173, -600, 25, 600)
+    model = read_mod(os.path.join(golden_dir, "obs.mod"))
+    rows = [l.split() for l in open(os.path.join(golden_dir, "synthesis_results.per"))]
+    assert len(rows) == 49
+    prev = O.use("liboracle_f32.so")
+    try:
+        s = O.make_setup(FE6173, TRIPLET, lam, psf=None, fix=[1] * 9 + [0, 1],
+                         guess_mac=model[9], slight=slight)
+        spec = O.synth(s, model).astype(np.float32)
+    finally:
+        O.use(prev)
+    mine = [["%e" % spec[st, i] for st in range(4)] for i in range(49)]
+    assert [r[2:] for r in rows] == mine
+
+
+def test_float_build_against_fp64():
+    """The float build's spectra agree with the FP64 build's to float
+    rounding (relative to the continuum), with the PSF and without."""
+    models = draw_models(16, seed=55)
+    for psf in (True, False):
+        s, lam, G = setup_l30(psf=psf)
+        ref = oracle_cube(s, models)
+        prev = O.use("liboracle_f32.so")
+        try:
+            s32, _, _ = setup_l30(psf=psf)
+            got = oracle_cube(s32, models)
+        finally:
+            O.use(prev)
+        err = np.abs(got - ref).max(axis=(1, 2)) / np.abs(ref[:, 0]).max(axis=1)
+        assert err.max() < 2e-6 and err.max() > 1e-9, err
+
+
 def test_primitives_match_reference_vectors():
     """fvoigt, Cuanten, fgauss_WL, fgauss and the convolutions against the
     reference's own code (vectors from tests/golden/make_ref_vectors.py)."""
