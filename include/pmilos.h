@@ -209,6 +209,16 @@ int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3]);
 /* Selects variant index (0 = the default mapping) of that form by hand. */
 int pm_ctx_set_variant(pm_ctx *ctx, int general, int index);
 
+/* No counterpart in the reference.  Selects a kernel shape of that form by
+ * its lanes per pixel, samples per lane and pixels per workgroup, among all
+ * the shapes the library has compiled that cover the context's sampling
+ * (lanes * samples >= L), including those the autotuner does not offer for
+ * this sampling (e.g. 64 x 2 at 30 samples); PM_ENOTSUP if there is none.
+ * For testing and tuning; pm_ctx_set_variant and pm_ctx_autotune replace
+ * the choice. */
+int pm_ctx_set_shape(pm_ctx *ctx, int general, int lanes, int samples,
+                     int pixels_per_workgroup);
+
 /* No counterpart in the reference: the mixed form's "chi2 unchanged" exit
  * test (milosUtils.c:579-580, |ochi2 - chi2| * 100 / chi2 < toplim) uses
  * max(toplim, toplim_mixed).  A chi2 summed from FP32 spectra cannot

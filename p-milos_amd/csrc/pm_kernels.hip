@@ -1182,6 +1182,10 @@ struct pm_ctx {
     int ncu = 0;
     hipStream_t last_stream = nullptr;
     int variant[2] = {0, 0};   // kernel variant of the 9-row and general forms
+    // a shape chosen by pm_ctx_set_shape among all compiled ones, in place of
+    // variant[] (valid when forced[] is set)
+    bool forced[2] = {false, false};
+    int forced_shape[2][3] = {{0, 0, 0}, {0, 0, 0}};
     // recorded after every launch that uses the context's work-queue
     // counters, so that a launch on another stream waits for it (the
     // counters are reset at the start of each launch)
@@ -1585,6 +1589,44 @@ static int variants(const pm_ctx *ctx, bool gen, InvVariant *v)
                                       : variants_of<double, 9, false>(L, v);
 }
 
+// Every compiled shape of a form that covers L samples (W * NLPL >= L):
+// the variants of each sampling bucket from L up.
+static int all_shapes(const pm_ctx *ctx, bool gen, InvVariant *v, int cap)
+{
+    int n = 0;
+    const int buckets[5] = {8, 16, 32, 64, PM_MAX_LAMBDA};
+    for (int b = 0; b < 5; b++) {
+        if (buckets[b] < ctx->h.L) continue;
+        InvVariant w[PM_MAXVAR];
+        int nw;
+        if (gen)
+            nw = ctx->precision == PM_MIXED ? variants_of<float, 11, true>(buckets[b], w)
+                                            : variants_of<double, 11, true>(buckets[b], w);
+        else
+            nw = ctx->precision == PM_MIXED ? variants_of<float, 9, false>(buckets[b], w)
+                                            : variants_of<double, 9, false>(buckets[b], w);
+        for (int i = 0; i < nw && n < cap; i++) v[n++] = w[i];
+    }
+    return n;
+}
+
+// The shape in use for a form: the one pm_ctx_set_shape chose, else the
+// autotuner's / pm_ctx_set_variant's index into variants(), else the first.
+static InvVariant selected(const pm_ctx *ctx, bool gen)
+{
+    if (ctx->forced[gen]) {
+        InvVariant v[5 * PM_MAXVAR];
+        const int n = all_shapes(ctx, gen, v, 5 * PM_MAXVAR);
+        for (int i = 0; i < n; i++)
+            if (v[i].W == ctx->forced_shape[gen][0] && v[i].NLPL == ctx->forced_shape[gen][1] &&
+                v[i].BLK == ctx->forced_shape[gen][2])
+                return v[i];
+    }
+    InvVariant v[PM_MAXVAR];
+    const int nv = variants(ctx, gen, v);
+    return v[ctx->variant[gen] < nv ? ctx->variant[gen] : 0];
+}
+
 // The 9-row kernel covers NTERMS 9 without stray light, with the PSF by
 // zero-padded correlation; everything else takes the general one.
 static bool needs_general(const pm_ctx *ctx, bool slight_cube)
@@ -1600,10 +1642,7 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
     if (n == 0) return PM_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool gen = needs_general(ctx, false);
-    InvVariant v[PM_MAXVAR];
-    const int nv = variants(ctx, gen, v);
-    const int k = ctx->variant[gen] < nv ? ctx->variant[gen] : 0;
-    return v[k].sfn(ctx, d_models, n, d_spectra, d_rf, st);
+    return selected(ctx, gen).sfn(ctx, d_models, n, d_spectra, d_rf, st);
 }
 
 int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
@@ -1628,10 +1667,7 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
     a.fallbacks = d_fallbacks;
     a.slight = d_slight;
     const bool gen = needs_general(ctx, d_slight != nullptr);
-    InvVariant v[PM_MAXVAR];
-    const int nv = variants(ctx, gen, v);
-    const int k = ctx->variant[gen] < nv ? ctx->variant[gen] : 0;
-    return v[k].fn(ctx, a, st, share);
+    return selected(ctx, gen).fn(ctx, a, st, share);
 }
 
 int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
@@ -1771,6 +1807,7 @@ extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *
     (void)hipFree(buf);
     if (rc != PM_OK) return rc;
     ctx->variant[gen] = best;
+    ctx->forced[gen] = false;
     if (chosen) { chosen[0] = v[best].W; chosen[1] = v[best].NLPL; chosen[2] = v[best].BLK / v[best].W; }
     return PM_OK;
 }
@@ -1780,10 +1817,10 @@ extern "C" int pm_ctx_variant(const pm_ctx *ctx, int general, int out[3])
     if (!ctx || !out) return PM_EINVAL;
     InvVariant v[PM_MAXVAR];
     const int nv = variants(ctx, general != 0, v);
-    const int k = ctx->variant[general != 0] < nv ? ctx->variant[general != 0] : 0;
-    out[0] = v[k].W;
-    out[1] = v[k].NLPL;
-    out[2] = v[k].BLK / v[k].W;
+    const InvVariant u = selected(ctx, general != 0);
+    out[0] = u.W;
+    out[1] = u.NLPL;
+    out[2] = u.BLK / u.W;
     return nv;
 }
 
@@ -1812,7 +1849,26 @@ extern "C" int pm_ctx_set_variant(pm_ctx *ctx, int general, int index)
     const int nv = variants(ctx, general != 0, v);
     if (index < 0 || index >= nv) return PM_EINVAL;
     ctx->variant[general != 0] = index;
+    ctx->forced[general != 0] = false;
     return PM_OK;
+}
+
+extern "C" int pm_ctx_set_shape(pm_ctx *ctx, int general, int lanes, int samples,
+                                int pixels_per_workgroup)
+{
+    if (!ctx || lanes < 1 || samples < 1 || pixels_per_workgroup < 1) return PM_EINVAL;
+    const bool gen = general != 0;
+    InvVariant v[5 * PM_MAXVAR];
+    const int n = all_shapes(ctx, gen, v, 5 * PM_MAXVAR);
+    for (int i = 0; i < n; i++)
+        if (v[i].W == lanes && v[i].NLPL == samples && v[i].BLK == lanes * pixels_per_workgroup) {
+            ctx->forced[gen] = true;
+            ctx->forced_shape[gen][0] = lanes;
+            ctx->forced_shape[gen][1] = samples;
+            ctx->forced_shape[gen][2] = v[i].BLK;
+            return PM_OK;
+        }
+    return PM_ENOTSUP;
 }
 
 // ------------------------------------------------------------ one launch for a batch

@@ -80,6 +80,7 @@ def lib():
         L.pm_ctx_autotune.argtypes = [vp, vp, vp, C.c_int, vp, C.POINTER(C.c_int * 3)]
         L.pm_ctx_variant.argtypes = [vp, C.c_int, C.POINTER(C.c_int * 3)]
         L.pm_ctx_set_variant.argtypes = [vp, C.c_int, C.c_int]
+        L.pm_ctx_set_shape.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.pm_ctx_set_mixed_toplim.argtypes = [vp, C.c_double]
         L.pm_graph_create.argtypes = [C.POINTER(BatchItem), C.c_int, C.c_double, C.POINTER(vp)]
         L.pm_graph_launch.argtypes = [vp, vp]
@@ -245,6 +246,11 @@ class Context:
 
     def set_variant(self, index, general=False):
         _check(lib().pm_ctx_set_variant(self.h, int(general), index))
+
+    def set_shape(self, lanes, samples, pixels_per_workgroup, general=False):
+        """Any compiled kernel shape that covers the sampling (pm_ctx_set_shape)."""
+        _check(lib().pm_ctx_set_shape(self.h, int(general), lanes, samples,
+                                      pixels_per_workgroup))
 
     def set_mixed_toplim(self, toplim_mixed):
         """The mixed form's chi2-unchanged threshold (reference units:

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import fitsutil
+import oracle as O
 import pmilos
 from fixtures import FE6173, FE6302, GOLDEN, GUESS, read_fits_2d, read_mod
 
@@ -50,6 +51,7 @@ def io():
     L.pmio_read_grid.argtypes = [C.c_char_p, vp, vp, vp]
     L.pmio_read_model.argtypes = [C.c_char_p, vp]
     L.pmio_interp_psf.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.c_double, vp]
+    L.pmio_read_psf_file.argtypes = [C.c_char_p, vp, vp, C.c_int]
     L.pmio_read_per.argtypes = [C.c_char_p, C.c_int, vp, vp]
     L.pmio_write_per.argtypes = [C.c_char_p, C.c_int, vp, vp, C.c_int]
     L.pmio_fits_read_cube.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
@@ -166,6 +168,40 @@ def test_psf_interpolation_is_linear(io):
                                   off.size, shift, G.ctypes.data) == 0
         want = np.interp(off - shift, dl, psf, left=0.0, right=0.0)
         np.testing.assert_allclose(G, want / want.sum(), rtol=1e-13, atol=1e-16)
+
+
+def crisp_psf_on_grid(initial, step, final):
+    """The reference's instrument profile run/CRISP_6173_25mA.psf (49
+    samples, mA, unit peak) on a wavelength grid, recentred as milos.c:331-340
+    does when the grid's zero offset is not its middle sample (shifted by
+    (L/2 - pos) steps) and normalised, by numpy's linear interpolation:
+    (offsets mA, shift mA, G)."""
+    dl, ps = np.loadtxt(os.path.join(GOLDEN, "CRISP_6173_25mA.psf")).T
+    lam = O.grid(FE6173, initial, step, final)
+    off = (lam - FE6173) * 1000.0
+    pos = int(np.flatnonzero(np.trunc(off) == 0)[0])
+    shift = (lam.size // 2 - pos) * step
+    G = np.interp(off - shift, dl, ps, left=0.0, right=0.0)
+    return off, shift, G / G.sum()
+
+
+def test_psf_file_recentred_like_the_reference(io):
+    """The file's samples, read by pmio_read_psf_file and interpolated by
+    pmio_interp_psf with the recentring shift, against numpy; the peak
+    lands on the grid's middle sample."""
+    dl = np.zeros(100)
+    ps = np.zeros(100)
+    n = io.pmio_read_psf_file(os.path.join(GOLDEN, "CRISP_6173_25mA.psf").encode(),
+                              dl.ctypes.data, ps.ctypes.data, 100)
+    want_dl, want_ps = np.loadtxt(os.path.join(GOLDEN, "CRISP_6173_25mA.psf")).T
+    assert n == 49 and np.array_equal(dl[:n], want_dl) and np.array_equal(ps[:n], want_ps)
+    off, shift, want = crisp_psf_on_grid(-300, 25, 500)
+    assert off.size == 33 and shift == 100.0
+    G = np.zeros(off.size)
+    assert io.pmio_interp_psf(dl.ctypes.data, ps.ctypes.data, n, off.ctypes.data, off.size,
+                              shift, G.ctypes.data) == 0
+    np.testing.assert_allclose(G, want, rtol=1e-13, atol=1e-17)
+    assert int(np.argmax(G)) == off.size // 2
 
 
 def test_per_round_trip(io, tmp_path):

@@ -89,6 +89,32 @@ def test_every_variant_matches_oracle(L, precision):
         cases.compare(ctx.invert(c.cube), name, form, say, cube=c.cube if k == 0 else None)
 
 
+@pytest.mark.parametrize("L", [30, 49])
+def test_fp64_two_sample_shape_at_small_samplings(L):
+    """The FP64 two-samples-per-lane shape (64 x 2, the only one offered in
+    FP64, for more than 64 samples) forced onto 30 and 49 samples, where
+    one sample per lane is the default: its inversions match the default
+    shape's at the FP64 floor tolerance (rounding-level agreement on every
+    parameter) on >= 99.5% of pixels, and the oracle at the case's
+    tolerance."""
+    name = "var_l%d" % L
+    c = cases.get(name)
+    ctx = c.context(pmilos)
+    base = ctx.invert(c.cube)
+    for ppw in (4, 2):
+        ctx.set_shape(64, 2, ppw)
+        assert ctx.variant()[0] == (64, 2, ppw)
+        say("case fp64 shape 64x2x%d %s" % (ppw, name))
+        g = ctx.invert(c.cube)
+        f = (np.abs(g["model"] - base["model"]) <= cases.FLOOR).mean(axis=0)
+        say("  within the floor of the default shape %s" % np.round(f, 4))
+        assert np.all(f >= cases.FRAC), f
+        cases.compare(g, name, "fp64", say)
+    # the shapes the library does not compile are refused
+    with pytest.raises(pmilos.PmError):
+        ctx.set_shape(16, 2, 8)
+
+
 def test_workgroup_size_does_not_change_bits():
     """Variants that differ only in pixels per workgroup give equal bits."""
     s, ctx, guess = _pair(30)

@@ -12,6 +12,7 @@ import time
 import numpy as np
 import pytest
 
+import cases
 import fitsutil
 import oracle as O
 import pmilos
@@ -159,6 +160,50 @@ def test_cli_fits_inversion_and_estimates(work):
         o = O.classical(s, cube[p], GUESS, for_init=False)
         np.testing.assert_allclose(est.reshape(13, -1)[[1, 2, 5, 6, 7], p],
                                    o[[1, 2, 5, 6, 7]].astype(np.float32), rtol=1e-6, atol=1e-5)
+
+
+def test_cli_psf_file_recentred(work):
+    """The PSF from the reference's instrument-profile file
+    run/CRISP_6173_25mA.psf on a grid (-300..500 mA, 25 mA, 33 samples)
+    whose zero offset is sample 12, not the middle one: the tool recentres
+    the profile by (L/2 - 12) steps (milos.c:320-343) and interpolates it
+    linearly (interpolationLinearPSF, milosUtils.c:684-738).  The cube is
+    synthesised by the oracle with the PSF that numpy interpolates from the
+    same file; the tool's inversions match the library called with that PSF
+    and the oracle."""
+    from test_cli_io import crisp_psf_on_grid
+    shutil.copy(os.path.join(GOLDEN, "CRISP_6173_25mA.psf"), work / "crisp.psf")
+    grid_file(str(work / "off.grid"), -300, 25, 500)
+    off, shift, G = crisp_psf_on_grid(-300, 25, 500)
+    assert shift == 100.0 and int(np.argmax(G)) == 16
+    lam = O.grid(FE6173, -300, 25, 500)
+    s = O.make_setup(FE6173, TRIPLET, lam, psf=G)
+    rows, cols = 8, 8
+    models = draw_models(rows * cols, 95)
+    cube = add_noise(np.stack([O.synth(s, m) for m in models]), models, 1e-3, 96)
+    fitsutil.write(str(work / "obs.fits"), fitsutil.stokes_cube(cube, rows, cols),
+                   ctypes=("WAVE-GRI", "STOKES", "HPLN-TAN", "HPLT-TAN"))
+    mtrol(str(work / "f.mtrol"), 50, "obs.fits", "off.grid", psf="crisp.psf")
+    run(["f.mtrol"], work)
+    planes = fitsutil.read(str(work / "obs_mod.fits"))[0].reshape(13, rows * cols)
+    cli = planes[:11].T.astype(np.float64)
+    ctx = pmilos.Context(FE6173, TRIPLET, lam, psf=G, guess=GUESS, noise=1e-6,
+                         ilambda=float(np.float32(0.1)), miter=50)
+    lib = ctx.invert(cube)["model"]
+    o = O.invert(s, cube, GUESS)["model"]
+    # the model cube holds float32; the PSFs of the tool and of numpy differ
+    # in the last bits at most
+    tol = np.maximum(cases.FLOOR, 1e-6 * np.abs(o))
+    near_lib = np.all(np.abs(cli - lib) <= np.maximum(tol, 1e-6 * np.abs(lib)), axis=1)
+    near_orc = np.all(np.abs(cli - o) <= np.maximum(tol, cases.tolerance("l30_config1")), axis=1)
+    say("psf file: %d/%d pixels match the library, %d/%d the oracle"
+        % (near_lib.sum(), near_lib.size, near_orc.sum(), near_orc.size))
+    assert near_lib.mean() >= 63 / 64 and near_orc.mean() >= 63 / 64
+    # a grid the file does not cover is refused
+    grid_file(str(work / "wide.grid"), -700, 25, 700)
+    mtrol(str(work / "w.mtrol"), 50, "obs.fits", "wide.grid", psf="crisp.psf")
+    r = subprocess.run([CLI, "w.mtrol"], cwd=work, capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "PSF" in (r.stdout + r.stderr)
 
 
 def test_cli_mixed_and_its_exit_threshold(work):
