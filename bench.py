@@ -16,11 +16,15 @@ results in host memory (PCIe copies included), and one pass of each form
 with the logclambda damping rule of run/pmilos.mtrol (logclambda = 1).
 The same field at noise sigma 1e-4 and 3e-3 Ic gives one more pass of each
 form (the mixed form's error against the FP64 form and the change in
-iteration count per sigma).  After these, a bounded streaming run (pmilos.serving:
-Poisson arrivals of scans of 32^2..2048^2 pixels at 6..112 wavelengths
-through the serving loop, mixed form) gives p50/p99 scan latency.
+iteration count per sigma).  The uneven batch (three cubes of 6, 30 and 112
+samples in one graph launch) follows, then a bounded streaming run
+(pmilos.serving: Poisson arrivals at half the calibrated capacity of 25
+scans of 1024^2 pixels at each of 6, 14, 30 and 112 wavelengths and 50 of
+32^2 / 256^2 pixels, mixed form) gives p50/p99 scan latency per kind.  The
+CPU baseline (the oracle's restatement, float and double builds, one
+process per core) runs first, before the GPU is touched.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--npix N] [--serve-scans S]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--npix N] [--serve-per-sampling S]
 """
 import argparse
 import json
@@ -71,11 +75,12 @@ def draw_models(n, seed):
 
 def _cpu_worker(args):
     """One core: synthesise its block of the workload with the oracle, add the
-    same-recipe noise, then time the oracle's inversion of the block."""
-    p0, p1, per_seed = args
-    os.environ["ORACLE_LIB"] = "liboracle_o3.so"
+    same-recipe noise, then time the inversion of the block by the oracle
+    build `build`."""
+    p0, p1, per_seed, build = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
+    O.use("liboracle.so")
     lam = O.grid(FE6173, -290, 20, 290)
     s = O.make_setup(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam))
     models = draw_models(p1, 2)[p0:p1]
@@ -83,25 +88,45 @@ def _cpu_worker(args):
     rng = np.random.default_rng(per_seed)
     ic = (models[:, 7] + models[:, 8])[:, None, None]
     stokes = (spec + rng.normal(0, 1, spec.shape) * 1e-3 * ic).astype(np.float32)
+    O.use(build)
+    s = O.make_setup(FE6173, TRIPLET, lam, psf=O.gauss_psf(60.0, lam))
     t = time.perf_counter()
     O.invert(s, stokes, GUESS)
     return p1 - p0, time.perf_counter() - t
 
 
+def cpu_cores():
+    """The host CPUs this process may use, at most 16 (a one-GPU job's share
+    of the GPU host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(per_core):
-    """The oracle (C restatement built -O3 -march=native) over equal pixel
-    blocks, one process per core, like pmilos.c:1181-1205.  Runs before the
-    GPU is initialised (plain fork)."""
-    cores = min(16, os.cpu_count() or 1)
+    """The oracle's restatement of the per-pixel path, built as the reference
+    builds itself (-O3 -march=native), over equal pixel blocks, one process
+    per core, like pmilos.c:1181-1205: the REAL = float build (the
+    reference's default, liboracle_f32.so; value) and the REAL = double one
+    (liboracle_o3.so).  Runs before the GPU is initialised (plain fork)."""
+    cores = cpu_cores()
     n = per_core * cores
-    blocks = [(i * per_core, (i + 1) * per_core, 3200 + i) for i in range(cores)]
-    with mp.get_context("fork").Pool(cores) as pool:
-        res = pool.map(_cpu_worker, blocks)
-    slowest = max(r[1] for r in res)
-    return dict(value=round(n / slowest, 1), unit="pixels/s", cores=cores, kind="port",
-                sample="first %d pixels of the workload (seed-2 models, same noise "
-                       "recipe), %d processes x %d pixels, slowest block %.1f s"
-                       % (n, cores, per_core, slowest))
+    out = {}
+    for name, build in (("float", "liboracle_f32.so"), ("fp64", "liboracle_o3.so")):
+        blocks = [(i * per_core, (i + 1) * per_core, 3200 + i, build) for i in range(cores)]
+        with mp.get_context("fork").Pool(cores) as pool:
+            res = pool.map(_cpu_worker, blocks)
+        slowest = max(r[1] for r in res)
+        out[name] = (round(n / slowest, 1), slowest)
+    return dict(value=out["float"][0], unit="pixels/s", cores=cores, kind="port",
+                per_core=round(out["float"][0] / cores, 1),
+                fp64_build={"value": out["fp64"][0], "per_core": round(out["fp64"][0] / cores, 1)},
+                sample="first %d pixels of the workload (seed-2 models, same noise recipe), "
+                       "%d processes x %d pixels; REAL=float build (the reference's default) "
+                       "slowest block %.1f s, REAL=double build %.1f s"
+                       % (n, cores, per_core, out["float"][1], out["fp64"][1]))
 
 
 def uneven_batch(args, torch, pmilos, precision):
@@ -109,7 +134,7 @@ def uneven_batch(args, torch, pmilos, precision):
     512x512 px at 30 (20 mA), 256x256 px of Fe I 6302.4936 A at 112 (21.5 mA)
     with v_mac ~ U(0,2) km/s free (10 parameters); model seeds 41/42/43,
     noise 1e-3 Ic.  Each sampling's kernel shape is picked by the autotuner
-    on its cube's first 32768 pixels; the batch is one HIP graph launch."""
+    on its whole cube; the batch is one HIP graph launch."""
     dev = torch.device("cuda:0")
     st = torch.cuda.current_stream()
     specs = [(512 * 512, (FE6173, -172.5, 69, 172.5), False, 41),
@@ -143,7 +168,7 @@ def uneven_batch(args, torch, pmilos, precision):
              torch.empty(n, dtype=torch.int32, device=dev)]
         shapes.append({"nlambda": L, "npix": n, "nterms": ctx.NT,
                        "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup":
-                           list(ctx.autotune_ptrs(cube.data_ptr(), min(n, 32768), st.cuda_stream))})
+                           list(ctx.autotune_ptrs(cube.data_ptr(), n, st.cuda_stream))})
         keep.append((ctx, cube, o))
         items.append(dict(ctx=ctx, d_stokes=cube.data_ptr(), npix=n, d_models=o[0].data_ptr(),
                           d_chi2=o[1].data_ptr(), d_iters=o[2].data_ptr(),
@@ -169,14 +194,40 @@ def uneven_batch(args, torch, pmilos, precision):
             it["ctx"].invert_ptrs(it["d_stokes"], it["npix"], it["d_models"], it["d_chi2"],
                                   it["d_iters"], it["d_accepted"], 0, st.cuda_stream)
     ms_seq = timed(one_by_one)
-    iters = [float(k[2][2].float().mean().item()) for k in keep]
     graph.close()
+    # each cube alone, for its own kernel time and roofline; the algorithmic
+    # work per pixel from its recorded trips (the closed-form FLOP model,
+    # with macroturbulence for the 112-sample cube)
+    from pmilos import flops
+    peak = PEAK_FP64_TFLOPS if precision == pmilos.PM_FP64 else PEAK_FP32_TFLOPS
+    total_flop = 0.0
+    for k, (it, shape, (n, g, mac, seed)) in enumerate(zip(items, shapes, specs)):
+        ms = timed(lambda it=it: it["ctx"].invert_ptrs(
+            it["d_stokes"], it["npix"], it["d_models"], it["d_chi2"], it["d_iters"],
+            it["d_accepted"], 0, st.cuda_stream))
+        iters = keep[k][2][2].cpu().numpy().astype(np.float64)
+        acc = keep[k][2][3].cpu().numpy().astype(np.float64)
+        fl = float(np.sum(flops.pixel_flops(iters, acc, L=shape["nlambda"], NT=shape["nterms"],
+                                            mac=mac)))
+        total_flop += fl
+        tf = fl / (ms * 1e-3) / 1e12
+        shape.update({"mean_iters": round(float(iters.mean()), 3),
+                      "mean_accepted": round(float(acc.mean()), 3),
+                      "kernel_ms_alone": round(ms, 3),
+                      "pixels_per_s_alone": round(n / (ms * 1e-3), 1),
+                      "roofline": {"bound": "valu", "achieved": round(tf, 3), "peak": peak,
+                                   "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                                   "flop_per_pixel": round(fl / n, 1)}})
+    tf = total_flop / (ms_graph * 1e-3) / 1e12
     return {"workload": "uneven batch: 512^2 px x 6 lambda + 512^2 px x 30 lambda + 256^2 px x "
                         "112 lambda (v_mac free), one graph launch",
             "pixels": total, "ms_per_launch": round(ms_graph, 3),
             "pixels_per_s": round(total / (ms_graph * 1e-3), 1),
             "ms_one_by_one": round(ms_seq, 3), "cubes": shapes,
-            "mean_iters": [round(x, 3) for x in iters]}
+            "roofline": {"bound": "valu", "achieved": round(tf, 3), "peak": peak,
+                         "unit": "TFLOP/s", "frac": round(tf / peak, 4),
+                         "note": "algorithmic FLOPs of the three cubes (closed-form model "
+                                 "over each pixel's recorded trips) / graph launch time"}}
 
 
 # the mixed form's stated per-parameter tolerance against the FP64 form
@@ -228,8 +279,10 @@ def main():
                     help="first run a plain device copy of this many MiB (the reference "
                          "kernel for scaling HBM traffic counters)")
     ap.add_argument("--cpu-per-core", type=int, default=4000)
-    ap.add_argument("--serve-scans", type=int, default=32,
-                    help="scans of the streaming request loop (0 = skip it)")
+    ap.add_argument("--serve-per-sampling", type=int, default=25,
+                    help="1024^2 scans per sampling in the streaming request loop (0 = skip it)")
+    ap.add_argument("--serve-small", type=int, default=50,
+                    help="32^2 and 256^2 scans besides them")
     ap.add_argument("--serve-util", type=float, default=0.5)
     ap.add_argument("--no-autotune", dest="autotune", action="store_false",
                     help="keep the default kernel shape instead of the autotuner's")
@@ -292,8 +345,10 @@ def main():
         cube = stokes if cube is None else cube
         steps = args.steps if steps is None else steps
         warmup = args.warmup if warmup is None else warmup
-        # kernel shape by the autotuner, on the first 65536 pixels (untimed)
-        shape = c.autotune_ptrs(cube.data_ptr(), min(npix, 65536), sptr) \
+        # kernel shape by the autotuner, on the whole cube (untimed): on a few
+        # times the pixels the GPU holds at once the persistent grid's tail
+        # hides the differences between the shapes
+        shape = c.autotune_ptrs(cube.data_ptr(), npix, sptr) \
             if args.autotune and tune else c.variant()[0]
 
         def step():
@@ -323,9 +378,9 @@ def main():
                     fallbacks=c.fallback_count(), models=out_m.cpu().numpy(), iters=iters)
 
     def host_to_host(c):
-        """One pass from the float32 cube in pinned host memory to the results
+        """Passes from the float32 cube in pinned host memory to the results
         in pinned host memory (upload, inversion, copy-back on one stream),
-        wall clock; and the copies alone."""
+        wall clock averaged over --steps passes; and the copies alone."""
         h_in = torch.empty(stokes.shape, dtype=torch.float32, pin_memory=True)
         h_in.copy_(stokes)
         d_in = torch.empty_like(stokes)
@@ -349,13 +404,15 @@ def main():
             fn()
             torch.cuda.synchronize()
             t = time.perf_counter()
-            fn()
+            for _ in range(args.steps):
+                fn()
             torch.cuda.synchronize()
-            res[name] = time.perf_counter() - t
+            res[name] = (time.perf_counter() - t) / args.steps
         del d_in, h_in, h_out
         return {"pixels_per_s": round(npix / res["whole"], 1),
                 "ms": round(res["whole"] * 1e3, 3),
-                "copies_only_ms": round(res["copies_only"] * 1e3, 3)}
+                "copies_only_ms": round(res["copies_only"] * 1e3, 3),
+                "passes": args.steps}
 
     def issued(r, form):
         # counter-measured issue rate of this pass (the per-pixel figures are
@@ -377,14 +434,14 @@ def main():
                                "sigma 1e-3 Ic, 9 free, <=50 LM it, plain FP64 form",
                    "npix": npix, "nlambda": L, "nterms": 9, "form": "fp64",
                    "lanes_per_pixel_x_samples_per_lane_x_pixels_per_workgroup": r64["shape"]},
-        "roofline": {"bound": "mfma", "achieved": round(r64["achieved"], 3),
+        "roofline": {"bound": "valu", "achieved": round(r64["achieved"], 3),
                      "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(r64["frac"], 4),
                      "traffic": round(TRAFFIC_B_PER_PX["fp64"] * npix),
                      "issued_fp64_tflops": round(issued(r64, "fp64")["fp64"], 3),
                      "issued_frac": round(issued(r64, "fp64")["fp64"] / PEAK_FP64_TFLOPS, 4),
-                     "note": "FP64 vector ALU work (the MI355X FP64 vector and matrix "
-                             "peaks are both 78.6 TFLOP/s); achieved = algorithmic "
+                     "note": "FP64 vector ALU work (78.6 TFLOP/s, the MI355X FP64 vector "
+                             "peak, equal to its FP64 matrix peak); achieved = algorithmic "
                              "FLOPs of the pass / pm_invert_kernel time; issued = lane "
                              "FLOPs issued per pixel from the SQ_INSTS_VALU_FLOPS "
                              "counters / the same time; traffic = HBM "
@@ -466,12 +523,13 @@ def main():
     if args.batch:
         line["uneven_batch"] = {"fp64": uneven_batch(args, torch, pmilos, pmilos.PM_FP64),
                                 "mixed": uneven_batch(args, torch, pmilos, pmilos.PM_MIXED)}
-    if args.serve_scans > 0:
+    if args.serve_per_sampling > 0:
         from pmilos import serving
         del stokes, out_m, out_c, out_i, out_a
         torch.cuda.empty_cache()
-        line["serving"] = serving.serve_bench(pmilos.PM_MIXED, args.serve_scans,
-                                              args.serve_util)
+        line["serving"] = serving.serve_bench(
+            pmilos.PM_MIXED, util=args.serve_util,
+            kinds=serving.bench_kinds(args.serve_per_sampling, args.serve_small))
     if cpu is not None:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

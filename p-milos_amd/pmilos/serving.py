@@ -7,11 +7,14 @@ Scan mix: sizes {32x32, 256x256, 1024x1024, 2048x2048} pixels and samplings
 {6, 14, 30, 112} wavelengths (Fe I 6173.3356 A at -172.5..172.5 @ 69 mA,
 -260..260 @ 40 mA, -290..290 @ 20 mA; Fe I 6302.4936 A at -1193.25..1193.25
 @ 21.5 mA with v_mac free, 10 parameters), Gaussian PSF of 60 mA, models
-drawn as the benchmark field's (seed 500 + scan kind), noise 1e-3 Ic.  One
-synthetic cube per (size, sampling) is made on the GPU and reused by the
-scans of that kind.
+drawn as the benchmark field's, noise 1e-3 Ic, arrivals from seed 50.  Scan
+k is synthesised from seed 500 + k (models and noise, made on the GPU) for
+the first few scans of each (size, sampling) kind -- four of each kind up to
+256^2 pixels, two of the larger ones -- and later scans of a kind reuse
+those cubes in turn (a 2048^2 x 112 cube is 7.5 GB of host memory).
 
-    python -m pmilos.serving [--scans N] [--util U] [--precision mixed|fp64]
+    python -m pmilos.serving [--scans N] [--util U ...] [--precision mixed|fp64]
+    python -m pmilos.serving --per-sampling 25 --small 50   (the benchmark's mix)
 """
 import argparse
 import json
@@ -79,12 +82,32 @@ def make_cube(ctx, npix, seed, mac):
     return cube
 
 
-def calibrate(ctxs, pool):
-    """Pixels/s of each context alone on its largest pooled cube (a bounded
-    slice), used to size the arrival rate for a target utilisation."""
+def scan_cubes(ctxs, kinds, distinct_small=4, distinct_large=2):
+    """The cube of every scan (host float32 arrays, shared between scans of
+    a kind past its first distinct_* ones): scan k's own cube is made from
+    seed 500 + k."""
+    made = {}
+    count = {}
+    out = []
+    for k, (size, L) in enumerate(kinds):
+        lst = made.setdefault((size, L), [])
+        cap = distinct_small if size <= 256 else distinct_large
+        if len(lst) < cap:
+            lst.append(make_cube(ctxs[L], size * size, 500 + k, L == 112))
+        j = count.get((size, L), 0)
+        count[(size, L)] = j + 1
+        out.append(lst[j % len(lst)])
+    return out, made
+
+
+def calibrate(ctxs, made):
+    """Pixels/s of each context alone on the first cube of its largest kind
+    (a bounded slice), used to size the arrival rate for a target
+    utilisation."""
     rate = {}
     for L, ctx in ctxs.items():
-        cube = pool[(max(s for s, l in pool if l == L), L)]
+        size = max(s for s, l in made if l == L)
+        cube = made[(size, L)][0]
         n = min(len(cube), 262144 if L <= 30 else 65536)
         ctx.invert(cube[:4096])
         t = time.perf_counter()
@@ -93,10 +116,11 @@ def calibrate(ctxs, pool):
     return rate
 
 
-def run(server, ctxs, pool, kinds, rate_hz, seed=50, progress=False):
-    """Submits len(kinds) scans at Poisson arrival times; returns per-scan
-    (kind, npix, latency) and the wall time from first arrival to last
-    completion.  progress: a line on stderr every 50 completed scans."""
+def run(server, ctxs, cubes, kinds, rate_hz, seed=50, progress=False):
+    """Submits len(kinds) scans (scan k: cubes[k]) at Poisson arrival times;
+    returns per-scan (kind, npix, latency) and the wall time from first
+    arrival to last completion.  progress: a line on stderr every 50
+    completed scans."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(kinds))
     tickets = []
@@ -104,13 +128,13 @@ def run(server, ctxs, pool, kinds, rate_hz, seed=50, progress=False):
 
     def submitter():
         t_next = time.perf_counter()
-        for k, gap in zip(kinds, gaps):
+        for k, cube, gap in zip(kinds, cubes, gaps):
             t_next += gap
             d = t_next - time.perf_counter()
             if d > 0:
                 time.sleep(d)
             size, L = k
-            tickets.append((k, server.submit(ctxs[L], pool[k])))
+            tickets.append((k, server.submit(ctxs[L], cube)))
         done.set()
 
     t0 = time.perf_counter()
@@ -149,28 +173,43 @@ def summarize(res, wall):
     return out
 
 
+def bench_kinds(per_sampling=25, small=50, seed=50, samplings=tuple(SAMPLINGS)):
+    """The benchmark's scan list: per_sampling scans of 1024^2 pixels at
+    every sampling (so that each sampling's p99 rests on that many scans)
+    and `small` scans of 32^2 and 256^2 pixels at random samplings, in a
+    seeded random order."""
+    rng = np.random.default_rng(seed + 1)
+    kinds = [(1024, L) for L in samplings for _ in range(per_sampling)]
+    kinds += [(int(rng.choice([32, 256])), int(rng.choice(samplings))) for _ in range(small)]
+    return [kinds[i] for i in rng.permutation(len(kinds))]
+
+
 def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SIZES,
-                samplings=tuple(SAMPLINGS), progress=False):
+                samplings=tuple(SAMPLINGS), progress=False, kinds=None):
+    """A streaming run: `scans` scans of random kinds from sizes x samplings
+    (or the given list of (size, L) kinds) arriving as a Poisson process at
+    the rate that loads the GPU to `util` of its calibrated capacity."""
     ctxs = contexts(precision)
-    kinds_all = [(s, L) for s in sizes for L in samplings]
-    pool = {}
-    for i, (s, L) in enumerate(kinds_all):
-        pool[(s, L)] = make_cube(ctxs[L], s * s, 500 + i, L == 112)
-    rate = calibrate(ctxs, pool)
-    rng = np.random.default_rng(seed)
-    kinds = [kinds_all[j] for j in rng.integers(0, len(kinds_all), scans)]
-    mean_service = np.mean([s * s / rate[L] for s, L in kinds_all])
+    if kinds is None:
+        kinds_all = [(s, L) for s in sizes for L in samplings]
+        rng = np.random.default_rng(seed)
+        kinds = [kinds_all[j] for j in rng.integers(0, len(kinds_all), scans)]
+    cubes, made = scan_cubes(ctxs, kinds)
+    rate = calibrate(ctxs, made)
+    mean_service = float(np.mean([s * s / rate[L] for s, L in kinds]))
     arrival = util / mean_service
     srv = pmilos.Server()
     try:
-        srv.wait(srv.submit(ctxs[30], pool[(32, 30)]))   # first use of its streams
-        res, wall = run(srv, ctxs, pool, kinds, arrival, seed, progress)
+        k0 = min(made)                                   # first use of its streams
+        srv.wait(srv.submit(ctxs[k0[1]], made[k0][0][:1024]))
+        res, wall = run(srv, ctxs, cubes, kinds, arrival, seed, progress)
     finally:
         srv.close()
     out = summarize(res, wall)
     out.update({"arrival_rate_hz": round(arrival, 4), "target_utilisation": util,
-                "mean_service_s": round(float(mean_service), 4),
+                "mean_service_s": round(mean_service, 4),
                 "alone_pixels_per_s": {str(L): round(r, 1) for L, r in rate.items()},
+                "distinct_cubes": sum(len(v) for v in made.values()),
                 "precision": "mixed" if precision == pmilos.PM_MIXED else "fp64"})
     return out
 
@@ -181,12 +220,18 @@ def main():
     ap.add_argument("--util", type=float, nargs="+", default=[0.5])
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--seed", type=int, default=50)
+    ap.add_argument("--per-sampling", type=int, default=0,
+                    help="the benchmark's mix: this many 1024^2 scans per sampling")
+    ap.add_argument("--small", type=int, default=50,
+                    help="with --per-sampling: this many 32^2/256^2 scans besides")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
     prec = pmilos.PM_MIXED if args.precision == "mixed" else pmilos.PM_FP64
+    kinds = bench_kinds(args.per_sampling, args.small, args.seed) if args.per_sampling else None
     for u in args.util:
-        print(json.dumps(serve_bench(prec, args.scans, u, args.seed, progress=True)), flush=True)
+        print(json.dumps(serve_bench(prec, args.scans, u, args.seed, progress=True, kinds=kinds)),
+              flush=True)
 
 
 if __name__ == "__main__":

@@ -56,3 +56,16 @@ def test_noise_sweep_statistics():
     assert r["mean_abs_iter_change"] == 1.0
     assert r["mean_iters_fp64"] == 40.0 and r["mean_iters_mixed"] == 39.0
     assert r["fp64_pixels_per_s"] == n / 0.1 and r["mixed_pixels_per_s"] == n / 0.05
+
+
+def test_flop_model_closed_forms():
+    """The frozen per-pixel FLOP terms at the worked configurations:
+    30 and 6 samples with the PSF (NT 9), 112 samples with macroturbulence
+    and the PSF (NT 10)."""
+    sys.path.insert(0, os.path.join(ROOT, "p-milos_amd"))
+    from pmilos import flops
+    assert flops.terms(30) == {"init": 121592, "iter": 26703, "acc": 100902}
+    assert flops.terms(6) == {"init": 14144, "iter": 9783, "acc": 11094}
+    assert flops.terms(112, NT=10, mac=True) == {"init": 1300707, "iter": 182214,
+                                                  "acc": 1124563}
+    assert flops.pixel_flops(25, 15) == 121592 + 25 * 26703 + 15 * 100902
