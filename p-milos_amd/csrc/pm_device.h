@@ -727,6 +727,156 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
     return tr < 1e9;
 }
 
+// The lanes of a group (as a bit mask, bit t for lane t) for which pred holds.
+template <int W>
+__device__ __forceinline__ unsigned long long group_ballot(bool pred)
+{
+    const unsigned long long b = __ballot(pred);
+    if (W == 64) return b;
+    const int base = __lane_id() & ~(W - 1);
+    return (b >> base) & ((1ull << W) - 1);
+}
+
+// Marks v as used here, so that a load feeding it is issued unconditionally
+// rather than moved behind the lane predicate that consumes it (a branch
+// per load).
+__device__ __forceinline__ void pin(double &v) { asm volatile("" : "+v"(v)); }
+
+// LDS doubles the lane-parallel solve uses: one slot per lane of the group
+// (pivot, column products, trace terms), 1/d (N), the factor's rows and
+// L^-1's rows (N x N each, row-major).
+template <int W, int N>
+struct LdlWs {
+    static constexpr int COL = 0, ID = W, F = W + N, M = F + N * N;
+    static constexpr int DOUBLES = M + N * N;
+};
+
+// ldl_solve with the factorisation and the trace bound spread over the
+// group's lanes: lane i < N holds row i of the damped matrix in registers
+// and step k of the factorisation publishes the pivot d_k and the products
+// l_jk d_k through LDS, after which every lane updates its own row
+// (right-looking).  Each entry of the lower triangle receives the same
+// operations in the same order as in ldl_solve (its products subtracted for
+// k ascending, then the scaling by 1/d_j), so the factor, delta and the
+// bound come out with the same bits.  The upper part of each lane's row
+// collects values nothing reads, which keeps the updates free of lane
+// predicates; lanes beyond N repeat row N-1.  The substitutions read the
+// factor from LDS, every lane running them (each lane needs all of delta);
+// L^-1 for the bound is formed a column per lane.  ws:
+// LdlWs<W, N>::DOUBLES doubles of LDS that the group may use; they are zero
+// again on return.  Needs W >= N.
+// Not called by the inversion kernel: the first GPU test run with it in the
+// kernel ended in a memory fault whose cause was not found (run on the CPU
+// with one thread per lane it is clean under AddressSanitizer and gives
+// ldl_solve's bits); tests/test_lanes_cpu.py holds it to those bits.
+template <int W, int N>
+__device__ __forceinline__ bool ldl_solve_lanes(const double *ab, double damp, double *x,
+                                                bool check, double *ws, int t)
+{
+    static_assert(W >= N, "one lane per row");
+    using WS = LdlWs<W, N>;
+    constexpr int P = N * (N + 1) / 2;
+    const int i = t < N ? t : N - 1;
+    const int ii = i * (i + 1) / 2;
+    double r[N];
+    const double di = ab[ii + i] * damp;                     // milosUtils.c:559-563
+    const bool act_i = di != 0.0;
+    const unsigned long long actm = group_ballot<W>(t < N && act_i) & ((1ull << N) - 1);
+    const bool all = actm == ((1ull << N) - 1);
+#pragma unroll
+    for (int k = 0; k < N; k++) { r[k] = ab[ii + (k < i ? k : i)]; pin(r[k]); }
+    // rows and columns of inactive parameters cleared, unit pivots for them
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        const bool keep = all || (act_i && ((actm >> k) & 1ull));
+        r[k] = k < i ? (keep ? r[k] : 0.0) : (k == i ? ((all || act_i) ? di : 1.0) : 0.0);
+    }
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+        ws[WS::COL + t] = r[k];
+        wave_sync();
+        const double dk = ws[WS::COL + k];
+        ok = ok && dk > 0.0;
+        const double inv = recip(dk);
+        ws[WS::ID + k] = inv;
+        r[k] = r[k] * inv;                       // l_ik below the pivot
+        const double w = r[k] * dk;
+        wave_sync();
+        ws[WS::COL + t] = w;
+        wave_sync();
+#pragma unroll
+        for (int j = k + 1; j < N; j++) {
+            double wj = ws[WS::COL + j];
+            pin(wj);
+            r[j] -= r[k] * wj;
+        }
+        wave_sync();
+    }
+    if (ok) {
+#pragma unroll
+        for (int k = 0; k < N; k++) ws[WS::F + i * N + k] = r[k];
+        wave_sync();
+        const double *F = ws + WS::F, *id = ws + WS::ID;
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+            double v = ((actm >> a) & 1ull) ? ab[P + a] : 0.0;
+#pragma unroll
+            for (int k = 0; k < a; k++) v -= F[a * N + k] * x[k];
+            x[a] = v;
+        }
+#pragma unroll
+        for (int a = 0; a < N; a++) x[a] *= id[a];
+#pragma unroll
+        for (int a = N - 1; a >= 0; a--) {
+            double v = x[a];
+#pragma unroll
+            for (int k = a + 1; k < N; k++) v -= F[k * N + a] * x[k];
+            x[a] = ((actm >> a) & 1ull) ? v : 0.0;
+        }
+        if (check) {
+            // column j = t of L^-1: m[a] = -(L_aj + sum_{j<k<a} L_ak m[k])
+            // for a > j, k ascending, as ldl_solve forms it in place (the
+            // terms k <= j add an exact 0: m[k] = 0 there)
+            const int j = i;
+            double m[N];
+#pragma unroll
+            for (int a = 0; a < N; a++) {
+                double v = F[a * N + j];
+#pragma unroll
+                for (int k = 1; k < a; k++) v += F[a * N + k] * m[k];
+                m[a] = a > j ? -v : 0.0;
+            }
+#pragma unroll
+            for (int a = 0; a < N; a++) ws[WS::M + a * N + j] = m[a];
+            wave_sync();
+            // row i: s_i = 1 + sum_{c<i} (L^-1)_ic^2 (zeros beyond the
+            // diagonal add nothing), then trace(A^-1) = sum_k s_k / d_k over
+            // the active rows, k ascending
+            double s = 1.0;
+#pragma unroll
+            for (int c = 0; c < N - 1; c++) {
+                const double q = ws[WS::M + i * N + c];
+                s += q * q;
+            }
+            const double tk = s * id[i];
+            wave_sync();
+            ws[WS::COL + t] = tk;
+            wave_sync();
+            double tr = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; k++) tr += ((actm >> k) & 1ull) ? ws[WS::COL + k] : 0.0;
+            ok = tr < 1e9;
+        }
+    }
+    wave_sync();
+#pragma unroll
+    for (int q = 0; q < (WS::DOUBLES + W - 1) / W; q++)
+        if (q * W + t < WS::DOUBLES) ws[q * W + t] = 0.0;
+    wave_sync();
+    return ok;
+}
+
 // Eigen pseudo-inverse by cyclic Jacobi rotations on a matrix kept in LDS,
 // the lanes of the group sharing the row/column updates.  Same rotation
 // formula, sweep order and stopping rule as the CPU restatement.  ws (LDS):
