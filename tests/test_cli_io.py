@@ -14,7 +14,10 @@ import oracle as O
 import pmilos
 from fixtures import FE6173, FE6302, GOLDEN, GUESS, read_fits_2d, read_mod
 
-IO_LIB = os.path.join(os.path.dirname(pmilos.LIB_PATH), "libpmilos_io.so")
+# PMIO_LIB: another build of the same library (the sanitizer run of
+# tests/test_host_sanitizers.py)
+IO_LIB = os.environ.get("PMIO_LIB") or os.path.join(os.path.dirname(pmilos.LIB_PATH),
+                                                    "libpmilos_io.so")
 P = 4096
 
 
