@@ -173,35 +173,51 @@ __device__ __forceinline__ float recip(float x) { return __builtin_amdgcn_rcpf(x
 template <typename T> struct AccT { typedef double type; };
 template <> struct AccT<float> { typedef float type; };
 
+// Integer sum / min / max and double max over the group, by the same DPP,
+// swizzle and readlane levels as group_sum (no LDS round trips): every lane
+// ends with the result.  Exact operations, so the order of the levels does
+// not change it.
+template <int W, typename F>
+__device__ __forceinline__ int group_reduce_int(int v, F op)
+{
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    v = op(v, dpp_i<0xB1>(v));
+    v = op(v, dpp_i<0x4E>(v));
+    v = op(v, dpp_i<0x141>(v));
+    if (W >= 16) v = op(v, dpp_i<0x140>(v));
+    if (W >= 32) v = op(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+    if (W >= 64) v = op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 32));
+    return v;
+}
+
 template <int W>
 __device__ __forceinline__ int group_sum_int(int v)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
-    return v;
+    return group_reduce_int<W>(v, [](int a, int b) { return a + b; });
 }
 
 template <int W>
 __device__ __forceinline__ int group_min_int(int v)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, W));
-    return v;
+    return group_reduce_int<W>(v, [](int a, int b) { return a < b ? a : b; });
 }
 
 template <int W>
 __device__ __forceinline__ int group_max_int(int v)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, W));
-    return v;
+    return group_reduce_int<W>(v, [](int a, int b) { return a > b ? a : b; });
 }
 
 template <int W>
 __device__ __forceinline__ double group_max(double v)
 {
-#pragma unroll
-    for (int o = W / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, W));
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    v = fmax(v, dpp_d<0xB1>(v));
+    v = fmax(v, dpp_d<0x4E>(v));
+    v = fmax(v, dpp_d<0x141>(v));
+    if (W >= 16) v = fmax(v, dpp_d<0x140>(v));
+    if (W >= 32) v = fmax(v, swz16_d(v));
+    if (W >= 64) v = fmax(readlane_d(v, 0), readlane_d(v, 32));
     return v;
 }
 

@@ -59,6 +59,19 @@ struct Lds {
 
 // ------------------------------------------------------------ convolution
 
+// Lane `src` of the group's value of v: a readlane (scalar, no LDS trip)
+// when the group is the whole wave, else a group shuffle.
+template <int W, typename T>
+__device__ __forceinline__ T lane_of(T v, int src)
+{
+    if constexpr (W == 64) {
+        if constexpr (sizeof(T) == 8) return readlane_d(v, src);
+        else return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+    } else {
+        return __shfl(v, src, W);
+    }
+}
+
 // Zero-padded "same" correlation of R rows x[r][slot] with the PSF, in place
 // (convolution.c:54-78).  Rows 0 and (when R == 8) 4 are Stokes I rows and
 // use the continuum trick of convolution.c:90-113 with Ic = max(first, last)
@@ -82,7 +95,7 @@ __device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
         T last = x[r][0];
 #pragma unroll
         for (int k = 1; k < NLPL; k++) if (k == klast) last = x[r][k];
-        T a = __shfl(x[r][0], 0, W), b = __shfl(last, tlast, W);
+        T a = lane_of<W>(x[r][0], 0), b = lane_of<W>(last, tlast);
         ic[q] = (a > b) ? a : b;
     }
 #pragma unroll
@@ -182,7 +195,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
         T last = x[r][0];
 #pragma unroll
         for (int k = 1; k < NLPL; k++) if (k == klast) last = x[r][k];
-        T a = __shfl(x[r][0], 0, W), b = __shfl(last, tlast, W);
+        T a = lane_of<W>(x[r][0], 0), b = lane_of<W>(last, tlast);
         ic[q] = (a > b) ? a : b;
     }
 #pragma unroll
