@@ -269,11 +269,13 @@ struct InvVariant {
 template <typename T, int NT, bool G>
 static int variants_of(int L, InvVariant *v)
 {
-    // Several samples per lane only in the FP32 form: the FP64 kernels of
-    // those shapes need far more than 256 registers per lane (occupancy 1,
-    // never the fastest), and twice their inversions came out wrong on the
-    // GPU after changes elsewhere in the kernel while the same source gave
-    // right results in every other shape.  L > 64 needs them in FP64 too.
+    // Several samples per lane in the FP32 form; in the FP64 form, whose
+    // two-sample kernels need far more than 256 registers per lane
+    // (occupancy 1, never the fastest), only 8x2 after the one-sample shapes
+    // and 64x2 where nothing else fits.  The FP64 8x2 and 16x2 kernels once
+    // gave wrong inversions on the GPU; for 8x2 the in-place correlation's
+    // missing wait (a block longer than the half-width) explains it and is
+    // fixed, for 16x2 no cause was found and it stays out.
     constexpr bool MULTI = !G && sizeof(T) == 4;
     // In the FP32 form the two-sample shapes come first: on a million pixels
     // they invert 14 and 30 samples 1.2x faster than one sample per lane
@@ -285,6 +287,7 @@ static int variants_of(int L, InvVariant *v)
         if constexpr (MULTI) v[n++] = PM_V(T, 8, 2, NT, G, 128);
         v[n++] = PM_V(T, 16, 1, NT, G, 256);
         v[n++] = PM_V(T, 16, 1, NT, G, 128);
+        if constexpr (!G && sizeof(T) == 8) v[n++] = PM_V(T, 8, 2, NT, G, 128);
     } else if (L <= 32) {
         if constexpr (MULTI) {
             v[n++] = PM_V(T, 16, 2, NT, G, 256);

@@ -194,8 +194,16 @@ def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SI
         kinds_all = [(s, L) for s in sizes for L in samplings]
         rng = np.random.default_rng(seed)
         kinds = [kinds_all[j] for j in rng.integers(0, len(kinds_all), scans)]
+    t0 = time.perf_counter()
     cubes, made = scan_cubes(ctxs, kinds)
+    if progress:
+        print("%d distinct cubes ready, %.1f s" % (sum(len(v) for v in made.values()),
+                                                   time.perf_counter() - t0),
+              file=sys.stderr, flush=True)
     rate = calibrate(ctxs, made)
+    if progress:
+        print("calibrated: %s pixels/s alone" % {L: round(r) for L, r in rate.items()},
+              file=sys.stderr, flush=True)
     mean_service = float(np.mean([s * s / rate[L] for s, L in kinds]))
     arrival = util / mean_service
     srv = pmilos.Server()
