@@ -298,6 +298,30 @@ template <typename T>
 __device__ __forceinline__ void make_geom(PmC c, const double *m,
                                           Geom<T> &g)
 {
+    if constexpr (sizeof(T) == 4) {
+        // the FP32 form: its spectra are FP32, and so are their angle terms
+        // (the reference's default build computes them in float too)
+        const float CC = float(PM_PI / 180.0), CC_2 = CC * 2.0f;
+        float sgm, cgm, sa, ca;
+        sincosf(float(m[5]) * CC, &sgm, &cgm);
+        sincosf(2.0f * (float(m[6]) * CC), &sa, &ca);
+        const float sinis = sgm * sgm, cosis = cgm * cgm;
+        g.sin_gm = sgm; g.cosi = cgm;
+        g.sinis = sinis; g.cosis = cosis; g.cosis_2 = (1.0f + cosis) / 2.0f;
+        g.sina = sa; g.cosa = ca;
+        g.sinda = ca * CC_2; g.cosda = -sa * CC_2;
+        g.sindi = cgm * sgm * CC_2; g.cosdi = -sgm * CC;
+        g.sinis_cosa = sinis * ca; g.sinis_sina = sinis * sa;
+        const float E0 = float(m[0]), LD = float(m[3]);
+        g.E0 = E0;
+        g.LD = LD; g.A = float(m[4]); g.MF = float(m[1]);
+        g.invE0 = 1.0f / E0;
+        g.invLD = 1.0f / LD;
+        g.B0 = float(m[7]); g.B1 = -(float(m[8]) * float(c->mu));
+        g.ulos = (float(m[2]) * float(c->lambda0)) / (float(PM_VLIGHT) * LD);
+        g.parcial = (float(c->lambda0 * c->lambda0) / LD) * float(PM_CTE4_6_13);
+        return;
+    }
     const double CC = PM_PI / 180.0, CC_2 = CC * 2;      // milos.c:232-233
     double GM = m[5] * CC, AZI = m[6] * CC;
     double sgm, cgm, sa, ca;

@@ -461,30 +461,35 @@ template <typename T, int W, int NLPL>
 __device__ __forceinline__ void mac_kernels(PmC c, const double *m,
                                             T *pad, int t, int L, T *gm, T *gd, Taps &tp)
 {
-    const double MC = m[9];
-    const double ild = (c->lambda0 * MC) / 2.99792458e5;
+    // in the arithmetic of the form (FP32 in the mixed form, as the
+    // reference's default build); the wavelength offset from the centre is
+    // formed in FP64 in both
+    typedef typename AccT<T>::type A;
+    const A MC = A(m[9]);
+    const A ild = (A(c->lambda0) * MC) / A(2.99792458e5);
     const double centro = c->lam[L / 2];
-    double gv[NLPL], a2[NLPL], part = 0;
+    A gv[NLPL], a2[NLPL];
+    double part = 0;
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
         gv[k] = 0;
         a2[k] = 0;
         if (lam < L) {
-            double a = (c->lam[lam] - centro) / ild;
-            double term = (a * a) / 2;
-            gv[k] = term < 1e30 ? exp(-term) : 0.0;
+            A a = A(c->lam[lam] - centro) / ild;
+            A term = (a * a) / A(2);
+            gv[k] = term < A(1e30) ? exp(-term) : A(0);
             a2[k] = a * a;
-            part += gv[k];
+            part += (double)gv[k];
         }
     }
-    const double sum = group_sum<W>(part);
+    const A sum = A(group_sum<W>(part));
     T x[2][NLPL];
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
-        double g = gv[k] / sum;
+        A g = gv[k] / sum;
         x[0][k] = T(g);
-        x[1][k] = T(g / MC * (a2[k] - 1.0));
+        x[1][k] = T(g / MC * (a2[k] - A(1)));
     }
     if (c->use_psf) convPSF<T, W, NLPL, 2, true, false>(c, pad, t, L, x);
 #pragma unroll
