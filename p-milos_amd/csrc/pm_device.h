@@ -361,6 +361,10 @@ __device__ __forceinline__ void profiles(PmC c, const Geom<T> &g,
     // the wavelength offset is formed in double: in FP32 the 0.5 mA spacing
     // of floats near 6000 A would be a percent of a Doppler width
     T u = (T(dl) * g.invLD) - g.ulos;
+    // defined whatever the component counts, so that no value of the
+    // previous synthesis stays live in a register across the LM loop
+#pragma unroll
+    for (int q = 0; q < 3; q++) { s.uu[q] = u; s.H[q] = T(0); s.F[q] = T(0); }
     // pi components
     {
         T fi = 0, shi = 0;
@@ -766,6 +770,14 @@ __device__ __forceinline__ bool ldl_solve(const double *ab /* LDS: packed lower 
     }
     return tr < 1e9;
 }
+
+// v with its value hidden from the optimiser (no instruction is emitted).
+// The inversion kernel takes its lane index through this once per trip of its
+// persistent loop: otherwise every per-lane address and loop-invariant load
+// derived from the lane index is hoisted out of the loop and occupies
+// registers for the kernel's whole life (tens of VGPRs, the difference
+// between one and two waves per SIMD for the general form).
+__device__ __forceinline__ int opaque(int v) { asm volatile("" : "+v"(v)); return v; }
 
 // The lanes of a group (as a bit mask, bit t for lane t) for which pred holds.
 template <int W>

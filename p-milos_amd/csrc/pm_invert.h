@@ -548,7 +548,10 @@ __device__ __forceinline__ void synth_slots(PmC c, const double *m,
 #pragma unroll
         for (int s = 0; s < 4; s++) S[s][k] = s4[s];
     }
+    // every field of ex is defined on every call (whatever the branches
+    // below take), so that none stays live in a register across the LM loop
     ex.mac = false;
+    ex.tp = Taps{0, 0, 0, 0, 0.0};
     if (GEN) {
 #pragma unroll
         for (int k = 0; k < NLPL; k++)
@@ -562,6 +565,12 @@ __device__ __forceinline__ void synth_slots(PmC c, const double *m,
     } else if (c->use_psf) {
         convPSF<T, W, NLPL, 4, GEN>(c, pad, t, L, S);              // :362-402
     }
+    if (GEN) {
+#pragma unroll
+        for (int k = 0; k < NLPL; k++)
+#pragma unroll
+            for (int s = 0; s < 4; s++) ex.Ssl[s][k] = S[s][k];
+    }
     if (GEN && slp) {                                              // :405-412
         const T alf = T(m[10]);
 #pragma unroll
@@ -569,10 +578,8 @@ __device__ __forceinline__ void synth_slots(PmC c, const double *m,
             int lam = t + k * W;
             int li = lam < L ? lam : L - 1;
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                ex.Ssl[s][k] = S[s][k];
+            for (int s = 0; s < 4; s++)
                 S[s][k] = S[s][k] * alf + T(slp[s * L + li]) * (T(1) - alf);
-            }
         }
     }
 }
@@ -774,6 +781,20 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 
 // ------------------------------------------------------------ inversion kernel
 
+// Waves per SIMD each inversion kernel is compiled for (the registers per
+// lane it may use: 512 / waves): three for the one-sample 9-row FP32
+// kernels (168 registers, where LDS allows three workgroups per CU), two for
+// the other FP32 kernels and the one-sample 9-row FP64 ones, one for the
+// FP64 general and two-sample kernels (which need more than 256).  The FP32
+// general kernels spill a few dozen registers at two waves and are still
+// faster than at one (the 112-sample cube 1.6x).
+template <typename T, int W, int NLPL, bool GEN>
+struct InvOcc {
+    static constexpr int value =
+        (sizeof(T) == 4 && !GEN && NLPL == 1 && W >= 16) ? 3
+        : (sizeof(T) == 4 || (NLPL == 1 && !GEN)) ? 2 : 1;
+};
+
 // One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
 // loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
 //
@@ -784,7 +805,7 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // does, so alpha, beta and delta agree with the reference's NTERMS-sized
 // ones on the rows it has.  nfree uses the reference's NTERMS.
 template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
-__global__ void __launch_bounds__(BLK, ((NLPL == 1 || sizeof(T) == 4) && !GEN ? 2 : 1))
+__global__ void __launch_bounds__(BLK, (InvOcc<T, W, NLPL, GEN>::value))
 pm_invert_kernel(InvArgs a)
 {
     using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
@@ -792,7 +813,8 @@ pm_invert_kernel(InvArgs a)
     __shared__ __align__(16) char smem[LD::BLOCK_BYTES];
     PmC c = (PmC)a.c;
     const int L = c->L;
-    const int grp = threadIdx.x / W, t = threadIdx.x % W;
+    const int grp = threadIdx.x / W, t0 = threadIdx.x % W;
+    int t = t0;
     char *gb = smem + grp * LD::GROUP_BYTES;
     T *pad = (T *)gb;
     float *sobs = (float *)(gb + LD::U_BYTES);
@@ -824,6 +846,9 @@ pm_invert_kernel(InvArgs a)
     const float *slp = nullptr;
 
     while (true) {
+        // lane-derived values recomputed per trip (not in the 9-row FP64
+        // kernels, whose occupancy it does not change and which it slows)
+        if (GEN || sizeof(T) == 4) t = opaque(t0);
         bool first = false;
         if (!iterating) {
             if (cur >= end) {
