@@ -42,8 +42,9 @@ struct PmConst {
     // 1/sigma^2 = 1/noise, [1] for a missing sample (1/sigma^2 = 1e-21)
     double swt[2][4];
     // J^T J entries (packed lower alpha, then beta) of fixed parameters, a
-    // bit each: [0] for the 9-row layout, [1] for the 11-row one
-    unsigned int zbits[2][4];
+    // bit each: [0] for the 9-row layout, [1] for the 11-row one, [2] for
+    // the 10-row one (v_mac in row 9)
+    unsigned int zbits[3][4];
     double guess[11];
     double nup[PM_MAXC], wep[PM_MAXC];
     double nub[PM_MAXC], web[PM_MAXC], wer[PM_MAXC];

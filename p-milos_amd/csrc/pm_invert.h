@@ -711,9 +711,9 @@ __device__ __forceinline__ void rf_slots(PmC c, const double *m,
             D[k][2][6] = T(tw * (A)q);
         }
     }
-    if (GEN && slp) {
+    if (GEN && NT == 11 && slp) {
         const T alf = T(m[10]);
-        const int ff = NT == 11 ? 10 : (c->fix[9] ? 10 : 9);
+        const int ff = 10;                     // the filling factor's row
 #pragma unroll
         for (int k = 0; k < NLPL; k++) {
             int lam = t + k * W;
@@ -799,11 +799,14 @@ struct InvOcc {
 // loop's guess copy, classical estimates and NaN guards (milos.c:958-983).
 //
 // GEN = false: NTERMS 9 with neither macroturbulence nor stray light.
-// GEN = true (NT = 11): the reference's NTERMS 9, 10 or 11 in the 11-row
+// GEN = true, NT = 11: the reference's NTERMS 9, 10 or 11 in the 11-row
 // layout (v_mac in row 9, the filling factor in row 10); rows the
 // reference's run does not have are fixed rows here, zeroed like FijaACero
 // does, so alpha, beta and delta agree with the reference's NTERMS-sized
 // ones on the rows it has.  nfree uses the reference's NTERMS.
+// GEN = true, NT = 10: the reference's own 10-row layout of a run with v_mac
+// free and no stray light (v_mac in row 9, no filling-factor row): the same
+// code without the 11th row in the response functions, J^T J and the solve.
 template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
 __global__ void __launch_bounds__(BLK, (InvOcc<T, W, NLPL, GEN>::value))
 pm_invert_kernel(InvArgs a)
@@ -1043,7 +1046,7 @@ pm_invert_kernel(InvArgs a)
             // share of the lanes (the shares combined by a cross-row swizzle /
             // shuffle) and one lane per entry writes it for the solve.
             constexpr int CH = LD::CH, NV = NP + NT, NCH = (NV + CH - 1) / CH;
-            static_assert((NT == 9 || NT == 11) && 32 % CH == 0, "J^T J layout");
+            static_assert((NT == 9 || NT == 10 || NT == 11) && 32 % CH == 0, "J^T J layout");
             constexpr int PARTS = W / CH;
 #pragma unroll
             for (int ch = 0; ch < NCH; ch++) {
@@ -1087,7 +1090,7 @@ pm_invert_kernel(InvArgs a)
                 if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
                 const int e = ch * CH + vv;
                 // a chunk's bits lie in one word (CH divides 32)
-                const unsigned zw = c->zbits[NT == 11 ? 1 : 0][(ch * CH) >> 5];
+                const unsigned zw = c->zbits[NT == 11 ? 1 : NT == 10 ? 2 : 0][(ch * CH) >> 5];
                 if ((zw >> (e & 31)) & 1u) sum = 0.0;
                 if (part == 0 && e < NV) ab[e] = sum;
                 wave_sync();

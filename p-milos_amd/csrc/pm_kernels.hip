@@ -308,32 +308,41 @@ static int variants_of(int L, InvVariant *v)
 
 constexpr int PM_MAXVAR = 8;
 
-static int variants(const pm_ctx *ctx, bool gen, InvVariant *v)
+// The general form runs the 10-row kernel when the reference's layout has
+// 10 rows with v_mac in row 9 and no filling-factor row (v_mac free, no
+// stray light, fix[10] == 0), else the 11-row one.  The two lists have the
+// same shapes in the same order, so a variant index or a forced shape
+// means the same kernel shape in either.
+static bool ten_rows(const pm_ctx *ctx, bool slight_cube)
 {
-    const int L = ctx->h.L;
+    return ctx->h.NT == 10 && ctx->h.fix[10] == 0 && !ctx->h.use_slight && !slight_cube;
+}
+
+static int variants_at(const pm_ctx *ctx, bool gen, bool cube, int L, InvVariant *v)
+{
+    const bool mx = ctx->precision == PM_MIXED;
+    if (gen && ten_rows(ctx, cube))
+        return mx ? variants_of<float, 10, true>(L, v) : variants_of<double, 10, true>(L, v);
     if (gen)
-        return ctx->precision == PM_MIXED ? variants_of<float, 11, true>(L, v)
-                                          : variants_of<double, 11, true>(L, v);
-    return ctx->precision == PM_MIXED ? variants_of<float, 9, false>(L, v)
-                                      : variants_of<double, 9, false>(L, v);
+        return mx ? variants_of<float, 11, true>(L, v) : variants_of<double, 11, true>(L, v);
+    return mx ? variants_of<float, 9, false>(L, v) : variants_of<double, 9, false>(L, v);
+}
+
+static int variants(const pm_ctx *ctx, bool gen, InvVariant *v, bool cube = false)
+{
+    return variants_at(ctx, gen, cube, ctx->h.L, v);
 }
 
 // Every compiled shape of a form that covers L samples (W * NLPL >= L):
 // the variants of each sampling bucket from L up.
-static int all_shapes(const pm_ctx *ctx, bool gen, InvVariant *v, int cap)
+static int all_shapes(const pm_ctx *ctx, bool gen, InvVariant *v, int cap, bool cube = false)
 {
     int n = 0;
     const int buckets[5] = {8, 16, 32, 64, PM_MAX_LAMBDA};
     for (int b = 0; b < 5; b++) {
         if (buckets[b] < ctx->h.L) continue;
         InvVariant w[PM_MAXVAR];
-        int nw;
-        if (gen)
-            nw = ctx->precision == PM_MIXED ? variants_of<float, 11, true>(buckets[b], w)
-                                            : variants_of<double, 11, true>(buckets[b], w);
-        else
-            nw = ctx->precision == PM_MIXED ? variants_of<float, 9, false>(buckets[b], w)
-                                            : variants_of<double, 9, false>(buckets[b], w);
+        const int nw = variants_at(ctx, gen, cube, buckets[b], w);
         for (int i = 0; i < nw && n < cap; i++) v[n++] = w[i];
     }
     return n;
@@ -341,18 +350,19 @@ static int all_shapes(const pm_ctx *ctx, bool gen, InvVariant *v, int cap)
 
 // The shape in use for a form: the one pm_ctx_set_shape chose, else the
 // autotuner's / pm_ctx_set_variant's index into variants(), else the first.
-static InvVariant selected(const pm_ctx *ctx, bool gen)
+// cube: the inversion has a per-pixel stray-light cube.
+static InvVariant selected(const pm_ctx *ctx, bool gen, bool cube = false)
 {
     if (ctx->forced[gen]) {
         InvVariant v[5 * PM_MAXVAR];
-        const int n = all_shapes(ctx, gen, v, 5 * PM_MAXVAR);
+        const int n = all_shapes(ctx, gen, v, 5 * PM_MAXVAR, cube);
         for (int i = 0; i < n; i++)
             if (v[i].W == ctx->forced_shape[gen][0] && v[i].NLPL == ctx->forced_shape[gen][1] &&
                 v[i].BLK == ctx->forced_shape[gen][2])
                 return v[i];
     }
     InvVariant v[PM_MAXVAR];
-    const int nv = variants(ctx, gen, v);
+    const int nv = variants(ctx, gen, v, cube);
     return v[ctx->variant[gen] < nv ? ctx->variant[gen] : 0];
 }
 
@@ -396,7 +406,7 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
     a.fallbacks = d_fallbacks;
     a.slight = d_slight;
     const bool gen = needs_general(ctx, d_slight != nullptr);
-    return selected(ctx, gen).fn(ctx, a, st, share);
+    return selected(ctx, gen, d_slight != nullptr).fn(ctx, a, st, share);
 }
 
 int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
@@ -492,7 +502,7 @@ extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *
     }
     const bool gen = needs_general(ctx, d_slight != nullptr);
     InvVariant v[PM_MAXVAR];
-    const int nv = variants(ctx, gen, v);
+    const int nv = variants(ctx, gen, v, d_slight != nullptr);
     const size_t n = npix;
     void *buf = nullptr;
     if (hipMalloc(&buf, n * (11 * sizeof(double) + 3 * sizeof(int))) != hipSuccess)

@@ -113,10 +113,10 @@ static inline int pm_const_build(const pm_config *cfg, PmConst &c)
         c.swt[1][i] = sqrt(c.weight[i] * 1e-21);
     }
     // J^T J entries of fixed parameters, zeroed after the sums: the rows of
-    // FijaACeroDerivadasNoNecesarias (milosUtils.c:199-227); in the 9- and
-    // 11-row layouts row p is fixed when fix[p] == 0
-    for (int lay = 0; lay < 2; lay++) {
-        const int nt = lay ? 11 : 9, np = nt * (nt + 1) / 2;
+    // FijaACeroDerivadasNoNecesarias (milosUtils.c:199-227); in the 9-, 11-
+    // and 10-row layouts row p is fixed when fix[p] == 0
+    for (int lay = 0; lay < 3; lay++) {
+        const int nt = lay == 0 ? 9 : lay == 1 ? 11 : 10, np = nt * (nt + 1) / 2;
         for (int i = 0; i < nt; i++) {
             const bool fi = c.fix[i] == 0;
             for (int h = 0; h <= i; h++) {

@@ -6,7 +6,8 @@
 //   lanes_run IN OUT
 //
 // IN (little-endian): int32 hdr[16], then the mode's arrays (see read_*).
-//   mode 1, inversion: hdr = {1, W, NLPL, BLK, general, precision, npix, L,
+//   mode 1, inversion: hdr = {1, W, NLPL, BLK, kind (0 the 9-row kernel, 1 the
+//     11-row general one, 2 the 10-row general one), precision, npix, L,
 //     miter, logclambda, use_psf, use_fft, no_classical, slight profile,
 //     slight cube, want_fit}; int32 fix[11]; float64 {sl, ll, jl, su, lu, ju,
 //     lambda0, noise, ilambda, toplim, mu, weight[4], 0}; float64 guess[11];
@@ -98,18 +99,21 @@ struct Shape {
     int prec, W, NLPL, BLK, gen;
     InvFn fn;
 };
-#define PM_LS(T, P, W, NLPL, NT, G, BLK) \
-    Shape{P, W, NLPL, BLK, G, &invert<T, W, NLPL, NT, G, BLK>}
+// gen: 0 the 9-row kernel, 1 the 11-row general one, 2 the 10-row general one
+#define PM_LS(T, P, W, NLPL, NT, G, BLK, KIND) \
+    Shape{P, W, NLPL, BLK, KIND, &invert<T, W, NLPL, NT, G, BLK>}
 // the shapes the CPU tests run
 const Shape SHAPES[] = {
-    PM_LS(double, PM_FP64, 8, 1, 9, false, 128),
-    PM_LS(double, PM_FP64, 8, 2, 9, false, 128),
-    PM_LS(double, PM_FP64, 16, 2, 9, false, 128),
-    PM_LS(double, PM_FP64, 16, 1, 9, false, 128),
-    PM_LS(double, PM_FP64, 32, 1, 9, false, 128),
-    PM_LS(double, PM_FP64, 64, 2, 11, true, 128),
-    PM_LS(float, PM_MIXED, 16, 2, 9, false, 128),
-    PM_LS(float, PM_MIXED, 64, 2, 11, true, 128),
+    PM_LS(double, PM_FP64, 8, 1, 9, false, 128, 0),
+    PM_LS(double, PM_FP64, 8, 2, 9, false, 128, 0),
+    PM_LS(double, PM_FP64, 16, 2, 9, false, 128, 0),
+    PM_LS(double, PM_FP64, 16, 1, 9, false, 128, 0),
+    PM_LS(double, PM_FP64, 32, 1, 9, false, 128, 0),
+    PM_LS(double, PM_FP64, 64, 2, 11, true, 128, 1),
+    PM_LS(double, PM_FP64, 64, 2, 10, true, 128, 2),
+    PM_LS(float, PM_MIXED, 16, 2, 9, false, 128, 0),
+    PM_LS(float, PM_MIXED, 64, 2, 11, true, 128, 1),
+    PM_LS(float, PM_MIXED, 64, 2, 10, true, 128, 2),
 };
 
 int mode_invert(const int *h, FILE *fout)

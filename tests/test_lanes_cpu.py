@@ -44,7 +44,8 @@ def _case(L, n, seed, mac=False):
     return s, cfg, cube, guess
 
 
-# (shape = precision, lanes, samples per lane, threads, general), sampling,
+# (shape = precision, lanes, samples per lane, threads, kernel: 0 9-row, 1 11-row
+# general, 2 10-row general), sampling,
 # pixels, v_mac free
 KERNELS = [
     ((0, 8, 1, 128, False), 6, 16, False),
@@ -53,14 +54,17 @@ KERNELS = [
     ((0, 16, 2, 128, False), 30, 16, False),
     ((0, 32, 1, 128, False), 30, 16, False),
     ((1, 16, 2, 128, False), 30, 16, False),
-    ((0, 64, 2, 128, True), 112, 4, True),
-    ((1, 64, 2, 128, True), 112, 4, True),
+    ((0, 64, 2, 128, 1), 112, 4, True),
+    ((1, 64, 2, 128, 1), 112, 4, True),
+    ((0, 64, 2, 128, 2), 112, 4, True),
+    ((1, 64, 2, 128, 2), 112, 4, True),
 ]
 
 
 @pytest.mark.parametrize("shape,L,n,mac", KERNELS,
                          ids=["%s-%dx%d-%s-L%d" % ("fp64" if k[0][0] == 0 else "mixed", k[0][1],
-                                                   k[0][2], "gen" if k[0][4] else "9row", k[1])
+                                                   k[0][2], ("9row", "gen11", "gen10")[int(k[0][4])],
+                                                   k[1])
                               for k in KERNELS])
 def test_kernel_on_cpu_lanes_matches_oracle(built, shape, L, n, mac):
     s, cfg, cube, guess = _case(L, n, 960 + L, mac)
