@@ -6,13 +6,17 @@
 //   dispatcher  splits every admitted scan into pixel chunks of about the
 //               same estimated work and keeps each compute stream busy with
 //               one chunk, always taking the next chunk from the admitted
-//               scan with the least work left (shortest remaining work
-//               first): a small scan that arrives behind a large one waits
-//               for one chunk per stream, not for the large scan.  A chunk's
-//               Stokes samples are uploaded on its stream just before its
-//               kernel.  Chunks of different scans run at the same time: a
-//               persistent kernel's workgroups retire as its work queue
-//               drains and the next chunk's take their place.
+//               scan with the earliest deadline.  A scan's deadline is its
+//               arrival plus 1.5 times its estimated time alone on the GPU
+//               (its work over the work rate the dispatcher measures on the
+//               chunks it runs): a small scan that arrives behind a large
+//               one waits for one chunk per stream, not for the large scan,
+//               and a large scan is not held back indefinitely by a stream
+//               of smaller ones.  A chunk's Stokes samples are uploaded on
+//               its stream just before its kernel.  Chunks of different
+//               scans run at the same time: a persistent kernel's workgroups
+//               retire as its work queue drains and the next chunk's take
+//               their place.
 //   stager      admits queued scans while the device bytes of the admitted
 //               ones stay within the HBM budget (the first queued scan that
 //               fits; any scan when none is admitted), and copies the
@@ -94,6 +98,7 @@ struct Req {
     int npix = 0, L = 0, chunk = 0;
     size_t bytes = 0;
     double cost = 1;                     // estimated work per pixel
+    Clock::time_point deadline;          // the dispatcher serves the earliest first
     // device buffers, carved from one pool block
     char *blk = nullptr;
     float *d_stokes = nullptr, *d_slight = nullptr, *d_chi2 = nullptr, *d_fit = nullptr;
@@ -111,6 +116,10 @@ struct Chunk {
     int p0, px;
     hipEvent_t ev;
 };
+
+// A scan's deadline: its arrival plus SLACK times its estimated time alone
+// (at least MIN_SLACK_S).
+constexpr double SLACK = 1.5, MIN_SLACK_S = 0.02;
 
 struct Slot {
     hipStream_t s = nullptr;
@@ -135,6 +144,10 @@ struct pm_server {
     size_t budget = 0;
     int chunk_px = 0;
     double ref_cost = 1;                 // cost of the 30-sample, 9-parameter workload
+    // work (pm_ctx_cost units) the GPU completes per second while chunks
+    // are in flight, measured over windows of at least 50 ms of busy time
+    double rate = 0;
+    double win_work = 0, win_busy_s = 0;
     std::mutex mu;
     std::condition_variable cv;
     std::deque<Req *> queued;
@@ -297,27 +310,46 @@ void pm_server::dispatcher_loop()
 {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
+    Clock::time_point last = Clock::now();
+    bool was_busy = false;
     while (true) {
+        // the work rate: completed chunk work over the time chunks were in
+        // flight
+        const Clock::time_point now = Clock::now();
+        if (was_busy) win_busy_s += std::chrono::duration<double>(now - last).count();
+        last = now;
+        if (win_busy_s >= 0.05) {
+            const double r = win_work / win_busy_s;
+            rate = rate > 0 ? 0.75 * rate + 0.25 * r : r;
+            win_work = win_busy_s = 0;
+        }
         bool busy = false;
         for (auto &sl : slots) {
             if (sl.busy) {
                 hipError_t e = hipEventQuery(sl.cur.ev);
                 if (e == hipErrorNotReady) { busy = true; continue; }
                 if (e != hipSuccess) sl.cur.r->rc = PM_EDEVICE;
+                win_work += sl.cur.px * sl.cur.r->cost;
                 sl.spare.push_back(sl.cur.ev);
                 back.push_back(sl.cur);
                 sl.busy = false;
                 cv.notify_all();
             }
-            // next chunk: from the admitted scan with the least work left
+            // next chunk: from the admitted scan with the earliest deadline
+            // (the least work left among equal deadlines)
             Req *best = nullptr;
             double best_w = 0;
             for (Req *r : ready) {
-                double w = (r->npix - r->next_px) * r->cost;
-                if (!best || w < best_w) { best = r; best_w = w; }
+                const double w = (r->npix - r->next_px) * r->cost;
+                if (!best || r->deadline < best->deadline ||
+                    (r->deadline == best->deadline && w < best_w)) {
+                    best = r;
+                    best_w = w;
+                }
             }
             if (best && launch(sl, best, lk)) busy = true;
         }
+        was_busy = busy;
         if (stop && idle()) break;
         if (busy) {
             lk.unlock();
@@ -351,6 +383,8 @@ extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
     // per-pixel cost of the reference workload: 30 samples, 9 parameters,
     // a PSF of about 23 taps
     s->ref_cost = 30 * (400.0 + 10.0 * 9 * 23) + 10.0 * 729;
+    // the work rate until the first measurement: 5M pixels/s of it
+    s->rate = 5e6 * s->ref_cost;
     int rc = PM_OK;
     if (hipStreamCreateWithFlags(&s->xfer, hipStreamNonBlocking) != hipSuccess) rc = PM_EDEVICE;
     s->slots.resize(o.streams);
@@ -420,6 +454,10 @@ extern "C" int pm_submit(pm_server *s, pm_ctx *ctx, const float *stokes, const f
     r->chunk = (int)std::max(1024.0, std::min(1e9, s->chunk_px * s->ref_cost / r->cost));
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
+    const double alone_s = npix * r->cost / s->rate;
+    r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
+                                    std::chrono::duration<double>(
+                                        std::max(MIN_SLACK_S, SLACK * alone_s)));
     r->id = s->next_id++;
     s->all[r->id] = r;
     if (npix == 0) {

@@ -11,7 +11,8 @@ drawn as the benchmark field's, noise 1e-3 Ic, arrivals from seed 50.  Scan
 k is synthesised from seed 500 + k (models and noise, made on the GPU) for
 the first few scans of each (size, sampling) kind -- four of each kind up to
 256^2 pixels, two of the larger ones -- and later scans of a kind reuse
-those cubes in turn (a 2048^2 x 112 cube is 7.5 GB of host memory).
+those cubes in turn (a 2048^2 x 112 cube is 7.5 GB of host memory).  The
+cubes are in pinned host memory; the results land in pageable arrays.
 
     python -m pmilos.serving [--scans N] [--util U ...] [--precision mixed|fp64]
     python -m pmilos.serving --per-sampling 25 --small 50   (the benchmark's mix)
@@ -64,8 +65,11 @@ def contexts(precision):
 
 
 def make_cube(ctx, npix, seed, mac):
-    """Synthetic float32 cube [npix][4][L] in host memory: the library's FP64
-    synthesis of seeded models plus seeded noise, made on the GPU."""
+    """Synthetic float32 cube [npix][4][L] in pinned host memory (where an
+    acquisition pipeline's DMA buffers hold a scan; the server's uploads then
+    run at the link's rate instead of through staging copies): the
+    library's FP64 synthesis of seeded models plus seeded noise, made on
+    the GPU."""
     import torch
     dev = torch.device("cuda", torch.cuda.current_device())
     L = ctx.L
@@ -77,9 +81,10 @@ def make_cube(ctx, npix, seed, mac):
     gen.manual_seed(seed + 1)
     ic = (m[:, 7] + m[:, 8]).view(-1, 1, 1)
     spec += torch.randn(spec.shape, dtype=torch.float64, device=dev, generator=gen) * 1e-3 * ic
-    cube = spec.to(torch.float32).cpu().numpy()
+    host = torch.empty((npix, 4, L), dtype=torch.float32, pin_memory=True)
+    host.copy_(spec.to(torch.float32))
     del spec, m
-    return cube
+    return host.numpy()             # the array keeps the pinned tensor alive
 
 
 def scan_cubes(ctxs, kinds, distinct_small=4, distinct_large=2):
@@ -185,10 +190,11 @@ def bench_kinds(per_sampling=25, small=50, seed=50, samplings=tuple(SAMPLINGS)):
 
 
 def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SIZES,
-                samplings=tuple(SAMPLINGS), progress=False, kinds=None):
+                samplings=tuple(SAMPLINGS), progress=False, kinds=None, chunk_px=0):
     """A streaming run: `scans` scans of random kinds from sizes x samplings
     (or the given list of (size, L) kinds) arriving as a Poisson process at
-    the rate that loads the GPU to `util` of its calibrated capacity."""
+    the rate that loads the GPU to `util` of its calibrated capacity.
+    chunk_px: the server's chunk size (0: its default)."""
     ctxs = contexts(precision)
     if kinds is None:
         kinds_all = [(s, L) for s in sizes for L in samplings]
@@ -206,7 +212,7 @@ def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SI
               file=sys.stderr, flush=True)
     mean_service = float(np.mean([s * s / rate[L] for s, L in kinds]))
     arrival = util / mean_service
-    srv = pmilos.Server()
+    srv = pmilos.Server(chunk_px=chunk_px)
     try:
         k0 = min(made)                                   # first use of its streams
         srv.wait(srv.submit(ctxs[k0[1]], made[k0][0][:1024]))
@@ -218,6 +224,7 @@ def serve_bench(precision=pmilos.PM_MIXED, scans=48, util=0.5, seed=50, sizes=SI
                 "mean_service_s": round(mean_service, 4),
                 "alone_pixels_per_s": {str(L): round(r, 1) for L, r in rate.items()},
                 "distinct_cubes": sum(len(v) for v in made.values()),
+                "chunk_px": chunk_px or 65536,
                 "precision": "mixed" if precision == pmilos.PM_MIXED else "fp64"})
     return out
 
@@ -232,14 +239,17 @@ def main():
                     help="the benchmark's mix: this many 1024^2 scans per sampling")
     ap.add_argument("--small", type=int, default=50,
                     help="with --per-sampling: this many 32^2/256^2 scans besides")
+    ap.add_argument("--chunk-px", type=int, default=0,
+                    help="the server's chunk size in pixels of the 30-sample workload "
+                         "(0: its default, 65536)")
     args = ap.parse_args()
     import torch
     torch.cuda.set_device(0)
     prec = pmilos.PM_MIXED if args.precision == "mixed" else pmilos.PM_FP64
     kinds = bench_kinds(args.per_sampling, args.small, args.seed) if args.per_sampling else None
     for u in args.util:
-        print(json.dumps(serve_bench(prec, args.scans, u, args.seed, progress=True, kinds=kinds)),
-              flush=True)
+        print(json.dumps(serve_bench(prec, args.scans, u, args.seed, progress=True, kinds=kinds,
+                                     chunk_px=args.chunk_px)), flush=True)
 
 
 if __name__ == "__main__":
