@@ -265,9 +265,10 @@ void pm_graph_destroy(pm_graph *g);
  * device) as they arrive and inverts them concurrently.  Scans are admitted
  * while the device bytes of the admitted ones stay within hbm_budget (the
  * first queued scan that fits; any scan when none is admitted), uploaded,
- * split into pixel chunks that are run least-remaining-work first on
- * several streams, and their results are copied back into the caller's
- * buffers.  A scan's results are the same bits as pm_invert / pm_invert_sl
+ * split into pixel chunks that run on several streams, the next chunk from
+ * the admitted scan with the earliest deadline (its submission plus 1.5
+ * times its estimated time alone on the GPU), and their results are copied
+ * back into the caller's buffers.  A scan's results are the same bits as pm_invert / pm_invert_sl
  * give for it alone.  The caller keeps its buffers alive and unchanged
  * until pm_wait returns for the ticket.  One host thread may submit while
  * others wait. */
@@ -276,7 +277,8 @@ typedef struct pm_server pm_server;
 typedef struct {
     size_t hbm_budget;           /* device bytes of admitted scans; 0 = half of the free memory */
     int streams;                 /* compute streams, 1..8 (0 = 3) */
-    int chunk_px;                /* pixels per chunk at 30 wavelengths, scaled by 30/L (0 = 65536) */
+    int chunk_px;                /* chunk size: the work of this many pixels of the 30-sample,
+                                    9-parameter workload (0 = 65536) */
 } pm_server_opts;
 
 int  pm_server_create(const pm_server_opts *opts, pm_server **out);

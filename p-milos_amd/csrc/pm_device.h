@@ -109,10 +109,48 @@ __device__ __forceinline__ double readlane_d(double v, int lane)
                             __builtin_amdgcn_readlane(lo, lane));
 }
 
+// Orders LDS traffic between lanes of one wavefront (a group inside one
+// wave needs no workgroup barrier).
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// A group of 128 lanes is the two waves of its workgroup (one pixel per
+// workgroup): its LDS traffic is ordered by the workgroup barrier, and its
+// sums add the two waves' totals, exchanged through LDS, in a fixed order,
+// so that both waves hold the same bits.
+template <int W>
+__device__ __forceinline__ void group_sync()
+{
+    if constexpr (W > 64) __syncthreads();
+    else wave_sync();
+}
+
+// The values of wave 0 and wave 1 of a two-wave workgroup (v uniform within
+// each wave).
+template <typename V>
+__device__ __forceinline__ void wave_pair(V v, V &w0, V &w1)
+{
+    __shared__ V xs[2];
+    if ((threadIdx.x & 63) == 0) xs[threadIdx.x >> 6] = v;
+    __syncthreads();
+    w0 = xs[0];
+    w1 = xs[1];
+    __syncthreads();
+}
+
 template <int W>
 __device__ __forceinline__ double group_sum(double v)
 {
-    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    if constexpr (W == 128) {
+        double a, b;
+        wave_pair(group_sum<64>(v), a, b);
+        return a + b;
+    }
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64 || W == 128, "group width");
     v += dpp_d<0xB1>(v);                 // quad_perm [1,0,3,2]
     v += dpp_d<0x4E>(v);                 // quad_perm [2,3,0,1]
     v += dpp_d<0x141>(v);                // row_half_mirror
@@ -120,44 +158,6 @@ __device__ __forceinline__ double group_sum(double v)
     if (W >= 32) v += swz16_d(v);        // rows 0<->1, 2<->3
     if (W >= 64) v = readlane_d(v, 0) + readlane_d(v, 32);
     return v;
-}
-
-template <int W>
-__device__ __forceinline__ float group_xchg_f(float v, int level)
-{
-    int i = __float_as_int(v);
-    switch (level) {
-    case 0: i = dpp_i<0xB1>(i); break;
-    case 1: i = dpp_i<0x4E>(i); break;
-    case 2: i = dpp_i<0x141>(i); break;
-    case 3: i = dpp_i<0x140>(i); break;
-    default: i = __builtin_amdgcn_ds_swizzle(i, 0x401F); break;
-    }
-    return __int_as_float(i);
-}
-
-// Compensated FP32 group sum: a TwoSum butterfly carrying the rounding
-// error of every addition.  TwoSum is exact in either operand order, so all
-// lanes end with bit-identical (sum, error) pairs; returned as their double
-// sum.
-template <int W>
-__device__ __forceinline__ double group_sum_comp(float v)
-{
-    constexpr int LEVELS = W == 8 ? 3 : W == 16 ? 4 : 5;
-    float sv = v, e = 0.0f;
-#pragma unroll
-    for (int lv = 0; lv < LEVELS; lv++) {
-        float so = group_xchg_f<W>(sv, lv);
-        float eo = group_xchg_f<W>(e, lv);
-        float t = __fadd_rn(sv, so);
-        float bp = __fsub_rn(t, sv);
-        float err = __fadd_rn(__fsub_rn(sv, __fsub_rn(t, bp)), __fsub_rn(so, bp));
-        sv = t;
-        e = __fadd_rn(__fadd_rn(e, eo), err);
-    }
-    double r = (double)sv + (double)e;
-    if (W >= 64) r = readlane_d(r, 0) + readlane_d(r, 32);
-    return r;
 }
 
 // 1/x: in FP64 the hardware reciprocal refined by two Newton steps (within
@@ -181,7 +181,12 @@ template <> struct AccT<float> { typedef float type; };
 template <int W, typename F>
 __device__ __forceinline__ int group_reduce_int(int v, F op)
 {
-    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    if constexpr (W == 128) {
+        int a, b;
+        wave_pair(group_reduce_int<64>(v, op), a, b);
+        return op(a, b);
+    }
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64 || W == 128, "group width");
     v = op(v, dpp_i<0xB1>(v));
     v = op(v, dpp_i<0x4E>(v));
     v = op(v, dpp_i<0x141>(v));
@@ -212,7 +217,12 @@ __device__ __forceinline__ int group_max_int(int v)
 template <int W>
 __device__ __forceinline__ double group_max(double v)
 {
-    static_assert(W == 8 || W == 16 || W == 32 || W == 64, "group width");
+    if constexpr (W == 128) {
+        double a, b;
+        wave_pair(group_max<64>(v), a, b);
+        return fmax(a, b);
+    }
+    static_assert(W == 8 || W == 16 || W == 32 || W == 64 || W == 128, "group width");
     v = fmax(v, dpp_d<0xB1>(v));
     v = fmax(v, dpp_d<0x4E>(v));
     v = fmax(v, dpp_d<0x141>(v));
@@ -222,14 +232,6 @@ __device__ __forceinline__ double group_max(double v)
     return v;
 }
 
-// Orders LDS traffic between lanes of one wavefront (groups never span
-// waves, so no workgroup barrier is needed).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // ------------------------------------------------------------ Voigt
 
@@ -826,7 +828,7 @@ template <int W, int N>
 __device__ __forceinline__ bool ldl_solve_lanes(const double *ab, double damp, double *x,
                                                 bool check, double *ws, int t)
 {
-    static_assert(W >= N, "one lane per row");
+    static_assert(W >= N && W <= 64, "one lane per row, inside one wave");
     using WS = LdlWs<W, N>;
     constexpr int P = N * (N + 1) / 2;
     const int i = t < N ? t : N - 1;
@@ -943,7 +945,7 @@ __device__ __noinline__ void jacobi_pinv(double *ws, int t)
     const double *beta = ws + 2 * N * N;
     double *delta = ws + 2 * N * N + N;
     for (int i = t; i < N * N; i += W) V[i] = ((i / N) == (i % N)) ? 1.0 : 0.0;
-    wave_sync();
+    group_sync<W>();
     for (int sweep = 0; sweep < 100; sweep++) {
         double off = 0, diag = 0;
         for (int i = 0; i < N; i++) {
@@ -959,26 +961,26 @@ __device__ __noinline__ void jacobi_pinv(double *ws, int t)
                 double theta = (aqq - app) / (2.0 * apq);
                 double tt = (theta >= 0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
                 double cs = 1.0 / sqrt(tt * tt + 1.0), sn = tt * cs;
-                wave_sync();
+                group_sync<W>();
                 for (int k = t; k < N; k += W) {
                     double akp = A[k * N + p], akq = A[k * N + q];
                     A[k * N + p] = cs * akp - sn * akq;
                     A[k * N + q] = sn * akp + cs * akq;
                 }
-                wave_sync();
+                group_sync<W>();
                 for (int k = t; k < N; k += W) {
                     double apk = A[p * N + k], aqk = A[q * N + k];
                     A[p * N + k] = cs * apk - sn * aqk;
                     A[q * N + k] = sn * apk + cs * aqk;
                 }
-                wave_sync();
+                group_sync<W>();
                 if (t == 0) { A[p * N + q] = 0.0; A[q * N + p] = 0.0; }
                 for (int k = t; k < N; k += W) {
                     double vkp = V[k * N + p], vkq = V[k * N + q];
                     V[k * N + p] = cs * vkp - sn * vkq;
                     V[k * N + q] = sn * vkp + cs * vkq;
                 }
-                wave_sync();
+                group_sync<W>();
             }
         }
     }
@@ -993,7 +995,7 @@ __device__ __noinline__ void jacobi_pinv(double *ws, int t)
         }
         delta[i] = s;
     }
-    wave_sync();
+    group_sync<W>();
 }
 
 // ------------------------------------------------------------ classical estimates

@@ -19,8 +19,8 @@
 
 #define PM_BLOCK 256
 // threads per workgroup for W-lane pixel groups (W = 8 would need more than
-// 64 KiB of LDS for 32 groups)
-template <int W> struct BlockOf { static constexpr int value = W == 8 ? 128 : PM_BLOCK; };
+// 64 KiB of LDS for 32 groups; a 128-lane group is its own workgroup)
+template <int W> struct BlockOf { static constexpr int value = (W == 8 || W == 128) ? 128 : PM_BLOCK; };
 #define PM_CHUNK 4           // pixels taken per work-counter increment
 #define PM_NSLICE 8          // one work counter per XCD
 #define PM_CSTRIDE 32        // ints between counters (separate cache lines)
@@ -34,7 +34,9 @@ struct Lds {
     // rows fall in different LDS banks
     static constexpr int RS = PADW + 1;
     static constexpr int CH = W >= 16 ? 16 : W;
-    static constexpr int RED_BYTES = (CH + 1) * W * 8;   // lane stride CH+1: no bank conflicts
+    // lane stride CH+1: no bank conflicts; a 128-lane group adds CH doubles
+    // for the exchange of the two waves' sums
+    static constexpr int RED_BYTES = ((CH + 1) * W + (W > 64 ? CH : 0)) * 8;
     // rows per PSF pass: 8 (two parameters) when they fit in the space the
     // J^T J reduction needs anyway, else 4
     static constexpr int PAD_ROWS = 8 * RS * (int)sizeof(T) <= RED_BYTES ? 8 : 4;
@@ -55,16 +57,25 @@ struct Lds {
     static constexpr int GPB = BLOCK / W;
     static constexpr int BLOCK_BYTES = GPB * GROUP_BYTES;
     static_assert(BLOCK_BYTES <= 65536, "LDS per workgroup above 64 KiB");
+    static_assert(W <= 64 || BLK == W, "a 128-lane group is its own workgroup");
 };
 
 // ------------------------------------------------------------ convolution
 
-// Lane `src` of the group's value of v: a readlane (scalar, no LDS trip)
-// when the group is the whole wave, else a group shuffle.
+// Lane `src` of the group's value of v (src uniform): a readlane (scalar, no
+// LDS trip) when the group is the whole wave, a group shuffle inside a wave,
+// through LDS between workgroup barriers for a two-wave group.
 template <int W, typename T>
 __device__ __forceinline__ T lane_of(T v, int src)
 {
-    if constexpr (W == 64) {
+    if constexpr (W == 128) {
+        __shared__ T xv;
+        if ((int)threadIdx.x == src) xv = v;
+        __syncthreads();
+        const T r = xv;
+        __syncthreads();
+        return r;
+    } else if constexpr (W == 64) {
         if constexpr (sizeof(T) == 8) return readlane_d(v, src);
         else return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
     } else {
@@ -111,7 +122,7 @@ __device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
     const int jlo = sizeof(T) == 8 ? c->glo64 : c->glo32;
     const int jhi = sizeof(T) == 8 ? c->ghi64 : c->ghi32;
 #pragma unroll
@@ -135,7 +146,7 @@ __device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
 }
 
 // Half-width of the register-blocked correlation for rows of NX samples
@@ -211,7 +222,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
     {
         const int r = t / NB, b = t % NB;
         const T *src = pad + r * RS + half - HC + b * B;
@@ -231,7 +242,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
         for (int d = -HC; d <= HC; d++)
 #pragma unroll
             for (int o = 0; o < B; o++) acc[o] += g[HC + d] * (A)w[o + HC + d];
-        wave_sync();                       // every window loaded before any store
+        group_sync<W>();                       // every window loaded before any store
         const bool irow = IC && (r % 4) == 0;
         const T icr = (R == 8 && r == 4) ? ic[1] : ic[0];
         T *dst = pad + r * RS + half + b * B;
@@ -239,7 +250,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
         for (int o = 0; o < B; o++)
             if (b * B + o < L) dst[o] = irow ? T((A)icr - acc[o]) : T(acc[o]);
     }
-    wave_sync();
+    group_sync<W>();
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
@@ -247,7 +258,7 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
 #pragma unroll
             for (int r = 0; r < R; r++) x[r][k] = pad[r * RS + half + lam];
     }
-    wave_sync();
+    group_sync<W>();
 }
 
 // Circular convolution with the kernel h (length L: a kernel in LDS, or the
@@ -270,7 +281,7 @@ __device__ __forceinline__ void convC(T *pad, HP h, int jlo, int jhi, int t, int
 #pragma unroll
             for (int r = 0; r < R; r++) pad[r * RS + half + lam] = x[r][k];
     }
-    wave_sync();
+    group_sync<W>();
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int m = t + k * W;
@@ -292,7 +303,7 @@ __device__ __forceinline__ void convC(T *pad, HP h, int jlo, int jhi, int t, int
             for (int r = 0; r < R; r++) x[r][k] = T(acc[r]);
         }
     }
-    wave_sync();
+    group_sync<W>();
 }
 
 // The circular convolution of convC, register-blocked: the rows are stored
@@ -343,7 +354,7 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
     const int r = t / NB, b = t % NB, m0 = b * B;
     A acc[B];
 #pragma unroll
@@ -364,12 +375,12 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
     T *dst = pad + r * RS + TC;
 #pragma unroll
     for (int o = 0; o < B; o++)
         if (m0 + o < L) dst[m0 + o] = T(acc[o]);
-    wave_sync();
+    group_sync<W>();
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         const int lam = t + k * W;
@@ -377,7 +388,7 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
 #pragma unroll
             for (int r2 = 0; r2 < R; r2++) x[r2][k] = pad[r2 * RS + TC + lam];
     }
-    wave_sync();
+    group_sync<W>();
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         const int lam = t + k * W;
@@ -391,7 +402,7 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
             }
         }
     }
-    wave_sync();
+    group_sync<W>();
 }
 
 // The macroturbulence convolutions: register-blocked where the rows fit.
@@ -506,7 +517,7 @@ __device__ __forceinline__ void mac_kernels(PmC c, const double *m,
         if (lam >= tp.mlo && lam < tp.mhi) tsum += (double)x[0][k];
     }
     tp.msum = group_sum<W>(tsum);
-    wave_sync();
+    group_sync<W>();
 }
 
 // What the general form keeps from a synthesis for its response functions.
@@ -784,15 +795,16 @@ __global__ void pm_reset_counters(int *counters, unsigned long long *fallbacks)
 // Waves per SIMD each inversion kernel is compiled for (the registers per
 // lane it may use: 512 / waves): three for the one-sample 9-row FP32
 // kernels (168 registers, where LDS allows three workgroups per CU), two for
-// the other FP32 kernels and the one-sample 9-row FP64 ones, one for the
-// FP64 general and two-sample kernels (which need more than 256).  The FP32
+// the other FP32 kernels, the one-sample 9-row FP64 ones and the two-wave
+// (128-lane) FP64 general one, one for the other FP64 general and
+// two-sample kernels (which need more than 256).  The FP32
 // general kernels spill a few dozen registers at two waves and are still
 // faster than at one (the 112-sample cube 1.6x).
 template <typename T, int W, int NLPL, bool GEN>
 struct InvOcc {
     static constexpr int value =
         (sizeof(T) == 4 && !GEN && NLPL == 1 && W >= 16) ? 3
-        : (sizeof(T) == 4 || (NLPL == 1 && !GEN)) ? 2 : 1;
+        : (sizeof(T) == 4 || (NLPL == 1 && (!GEN || W > 64))) ? 2 : 1;
 };
 
 // One pixel per group: lm_mils (milosUtils.c:466-635) preceded by the pixel
@@ -829,7 +841,7 @@ pm_invert_kernel(InvArgs a)
     T *gm = (T *)(gb + LD::U_BYTES + LD::OBS_BYTES + LD::AB_BYTES + LD::LM_BYTES);
     T *gd = gm + W * NLPL;
     for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
-    wave_sync();
+    group_sync<W>();
 
     const int xcd = blockIdx.x % PM_NSLICE;
     int sl_off = 0, cur = 0, end = 0;
@@ -857,9 +869,15 @@ pm_invert_kernel(InvArgs a)
             if (cur >= end) {
                 int lo = -1, hi = -1;
                 if (t == 0) fetch_chunk(a.counters, xcd, a.npix, a.slice, sl_off, lo, hi);
-                lo = __shfl(lo, 0, W);
-                hi = __shfl(hi, 0, W);
-                sl_off = __shfl(sl_off, 0, W);
+                if constexpr (W > 64) {
+                    lo = lane_of<W>(lo, 0);
+                    hi = lane_of<W>(hi, 0);
+                    sl_off = lane_of<W>(sl_off, 0);
+                } else {
+                    lo = __shfl(lo, 0, W);
+                    hi = __shfl(hi, 0, W);
+                    sl_off = __shfl(sl_off, 0, W);
+                }
                 if (lo < 0) break;
                 cur = lo;
                 end = hi;
@@ -882,15 +900,15 @@ pm_invert_kernel(InvArgs a)
                     }
                 }
             }
-            wave_sync();
+            group_sync<W>();
             nfree = 4 * L - c->NT - group_sum_int<W>(ng);     // :507-508
             if (c->no_classical) {
                 if (t == 0)
                     for (int i = 0; i < 11; i++) lm[i] = lm[11 + i] = c->guess[i];
-                wave_sync();
+                group_sync<W>();
             } else {
                 Classic e = classical_group<W, NLPL>(c, sobs, true, t);
-                wave_sync();
+                group_sync<W>();
                 if (t == 0) {
                     for (int i = 0; i < 11; i++) lm[i] = c->guess[i];
                     lm[1] = e.B;
@@ -899,7 +917,7 @@ pm_invert_kernel(InvArgs a)
                     lm[6] = e.az;
                     for (int i = 0; i < 11; i++) lm[11 + i] = lm[i];
                 }
-                wave_sync();
+                group_sync<W>();
             }
             if (nfree == 0) {                                 // :509-512, milos.c:997
                 if (t == 0) {
@@ -938,13 +956,13 @@ pm_invert_kernel(InvArgs a)
                     ws[e] = (i == j) ? v * (1.0 + flambda) : v;
                 }
                 for (int i = t; i < NT; i += W) ws[2 * NT * NT + i] = ab[NP + i];
-                wave_sync();
+                group_sync<W>();
                 jacobi_pinv<W, NT>(ws, t);
 #pragma unroll
                 for (int i = 0; i < NT; i++) delta[i] = ws[2 * NT * NT + NT + i];
-                wave_sync();
+                group_sync<W>();
                 for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
-                wave_sync();
+                group_sync<W>();
                 if (t == 0) atomicAdd(a.fallbacks, 1ull);
             }
             {
@@ -953,10 +971,10 @@ pm_invert_kernel(InvArgs a)
                 for (int i = 0; i < 11; i++) { cur[i] = lm[i]; tr[i] = lm[11 + i]; }
                 aplica_delta(c, NT, cur, delta, tr);          // :566
                 check_model(tr);                              // :568
-                wave_sync();
+                group_sync<W>();
                 if (t == 0)
                     for (int i = 0; i < 11; i++) lm[11 + i] = tr[i];
-                wave_sync();
+                group_sync<W>();
             }
         }
 
@@ -993,7 +1011,7 @@ pm_invert_kernel(InvArgs a)
             if (acc) {                                         // :581-606
                 flambda = flambda / (5.0 * factor);
                 for (int i = t; i < 11; i += W) lm[i] = lm[11 + i];
-                wave_sync();
+                group_sync<W>();
                 ochisqr = c2;
                 nacc++;
             } else {                                           // :607-615
@@ -1074,7 +1092,7 @@ pm_invert_kernel(InvArgs a)
                         red[t * (CH + 1) + ib % CH] = (double)b;
                     }
                 }
-                wave_sync();
+                group_sync<W>();
                 const int vv = t % CH, part = t / CH;
                 constexpr int SPAN = W / PARTS;
                 // pairwise, so the adds form a tree of depth log2(SPAN)
@@ -1087,17 +1105,24 @@ pm_invert_kernel(InvArgs a)
                     for (int l = 0; l + w < SPAN; l += 2 * w) v[l] += v[l + w];
                 double sum = v[0];
                 if (PARTS >= 2) sum += swz16_d(sum);
-                if (PARTS >= 4) sum += __shfl_xor(sum, 32, W);
+                if (PARTS >= 4) sum += __shfl_xor(sum, 32, W > 64 ? 64 : W);
+                if constexpr (W > 64) {
+                    // wave 1's totals (parts 4..7) added to wave 0's
+                    double *xw = red + (CH + 1) * W;
+                    if (part == 4) xw[vv] = sum;
+                    group_sync<W>();
+                    if (part == 0) sum += xw[vv];
+                }
                 const int e = ch * CH + vv;
                 // a chunk's bits lie in one word (CH divides 32)
                 const unsigned zw = c->zbits[NT == 11 ? 1 : NT == 10 ? 2 : 0][(ch * CH) >> 5];
                 if ((zw >> (e & 31)) & 1u) sum = 0.0;
                 if (part == 0 && e < NV) ab[e] = sum;
-                wave_sync();
+                group_sync<W>();
             }
             for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
-            wave_sync();
-            wave_sync();
+            group_sync<W>();
+            group_sync<W>();
         }
 
         if (finished) {
@@ -1144,7 +1169,7 @@ pm_synth_kernel(const PmConst *c0, const double *models, int n, double *spectra,
     const int NTR = c->NT;
     const int row9 = (NTR == 10 && slp && !c->fix[9]) ? 10 : 9;
     for (int i = t; i < LD::PAD_ROWS * LD::RS; i += W) pad[i] = T(0);
-    wave_sync();
+    group_sync<W>();
     const int ngroups = gridDim.x * LD::GPB;
     for (int pix = blockIdx.x * LD::GPB + grp; pix < n; pix += ngroups) {
         double m[11];

@@ -250,10 +250,11 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
 // per lane NLPL (W * NLPL >= L) and threads per workgroup (pixels per
 // workgroup = threads / W).  The first of each list is the default mapping:
 // in the FP64 form the smallest power of two >= L lanes up to a full
-// wavefront, then two samples per lane; in the FP32 form two samples per
-// lane where such a shape is listed.  T = double (FP64 form) or float (mixed form: spectra,
-// RFs and convolution in FP32); the general form (NTERMS 10/11,
-// macroturbulence, stray light) is the 11-row kernel.  The synthesis kernel
+// wavefront, then a pixel over two waves (128 lanes, one sample each); in
+// the FP32 form two samples per lane where such a shape is listed.  T =
+// double (FP64 form) or float (mixed form: spectra, RFs and convolution in
+// FP32); the general form (NTERMS 10/11, macroturbulence, stray light) is
+// the 10- or 11-row kernel.  The synthesis kernel
 // follows the selected variant's lane mapping.
 typedef int (*InvLaunch)(pm_ctx *, const InvArgs &, hipStream_t, double);
 typedef int (*SynLaunch)(pm_ctx *, const double *, int, double *, double *, hipStream_t);
@@ -272,7 +273,7 @@ static int variants_of(int L, InvVariant *v)
     // Several samples per lane in the FP32 form; in the FP64 form, whose
     // two-sample kernels need far more than 256 registers per lane
     // (occupancy 1, never the fastest), only 8x2 after the one-sample shapes
-    // and 64x2 where nothing else fits.  The FP64 8x2 and 16x2 kernels once
+    // and, beyond 64 samples, 64x2 after the two-wave 128x1 shape.  The FP64 8x2 and 16x2 kernels once
     // gave wrong inversions on the GPU; for 8x2 the in-place correlation's
     // missing wait (a block longer than the half-width) explains it and is
     // fixed, for 16x2 no cause was found and it stays out.
@@ -300,6 +301,11 @@ static int variants_of(int L, InvVariant *v)
         v[n++] = PM_V(T, 64, 1, NT, G, 256);
         v[n++] = PM_V(T, 64, 1, NT, G, 128);
     } else {
+        // FP64: a pixel over two waves, one sample per lane, first: two waves
+        // per SIMD where the two-sample kernels fit only one (3% faster on
+        // the 112-sample cube with v_mac free; the workgroup barriers of
+        // its convolutions and sums take most of the gain)
+        if constexpr (sizeof(T) == 8) v[n++] = PM_V(T, 128, 1, NT, G, 128);
         v[n++] = PM_V(T, 64, 2, NT, G, 256);
         v[n++] = PM_V(T, 64, 2, NT, G, 128);
     }

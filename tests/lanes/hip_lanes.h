@@ -43,15 +43,20 @@ inline lanes_dim3 blockDim, gridDim;
 struct LanesGroup {
     int W = 0;
     std::barrier<> *bar = nullptr;
-    uint64_t slot[64];
+    uint64_t slot[128];
 };
 inline thread_local LanesGroup *lanes_grp = nullptr;
 
 inline int __lane_id() { return (int)(threadIdx.x % 64); }
 
-inline int lanes_in_group() { return __lane_id() % lanes_grp->W; }
+// the lane's index in its group (a group of 128 lanes spans two waves)
+inline int lanes_in_group() { return (int)(threadIdx.x % lanes_grp->W); }
 
 inline void lanes_barrier() { lanes_grp->bar->arrive_and_wait(); }
+
+// the workgroup barrier; the harness runs 128-lane groups as one workgroup
+// each, the only use the device code makes of it
+inline void __syncthreads() { lanes_barrier(); }
 
 // value of lane src_wave_lane (a lane of the same 64-lane wave) after all
 // lanes of the group have published theirs

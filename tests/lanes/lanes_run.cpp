@@ -111,6 +111,9 @@ const Shape SHAPES[] = {
     PM_LS(double, PM_FP64, 32, 1, 9, false, 128, 0),
     PM_LS(double, PM_FP64, 64, 2, 11, true, 128, 1),
     PM_LS(double, PM_FP64, 64, 2, 10, true, 128, 2),
+    PM_LS(double, PM_FP64, 128, 1, 10, true, 128, 2),
+    PM_LS(double, PM_FP64, 128, 1, 11, true, 128, 1),
+    PM_LS(double, PM_FP64, 128, 1, 9, false, 128, 0),
     PM_LS(float, PM_MIXED, 16, 2, 9, false, 128, 0),
     PM_LS(float, PM_MIXED, 64, 2, 11, true, 128, 1),
     PM_LS(float, PM_MIXED, 64, 2, 10, true, 128, 2),

@@ -90,21 +90,21 @@ def test_every_variant_matches_oracle(L, precision):
 
 
 @pytest.mark.parametrize("L", [30, 49])
-def test_fp64_two_sample_shape_at_small_samplings(L):
-    """The FP64 two-samples-per-lane shape (64 x 2, the only one offered in
-    FP64, for more than 64 samples) forced onto 30 and 49 samples, where
-    one sample per lane is the default: its inversions match the default
-    shape's at the FP64 floor tolerance (rounding-level agreement on every
-    parameter) on >= 99.5% of pixels, and the oracle at the case's
-    tolerance."""
+def test_fp64_wide_shapes_at_small_samplings(L):
+    """The FP64 shapes offered for more than 64 samples -- two samples per
+    lane (64 x 2) and a pixel over two waves (128 x 1, one sample per lane)
+    -- forced onto 30 and 49 samples, where one sample per lane in one wave
+    is the default: their inversions match the default shape's at the FP64
+    floor tolerance (rounding-level agreement on every parameter) on
+    >= 99.5% of pixels, and the oracle at the case's tolerance."""
     name = "var_l%d" % L
     c = cases.get(name)
     ctx = c.context(pmilos)
     base = ctx.invert(c.cube)
-    for ppw in (4, 2):
-        ctx.set_shape(64, 2, ppw)
-        assert ctx.variant()[0] == (64, 2, ppw)
-        say("case fp64 shape 64x2x%d %s" % (ppw, name))
+    for shape in ((64, 2, 4), (64, 2, 2), (128, 1, 1)):
+        ctx.set_shape(*shape)
+        assert ctx.variant()[0] == shape
+        say("case fp64 shape %dx%dx%d %s" % (shape + (name,)))
         g = ctx.invert(c.cube)
         f = (np.abs(g["model"] - base["model"]) <= cases.FLOOR).mean(axis=0)
         say("  within the floor of the default shape %s" % np.round(f, 4))

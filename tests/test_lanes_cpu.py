@@ -58,6 +58,9 @@ KERNELS = [
     ((1, 64, 2, 128, 1), 112, 4, True),
     ((0, 64, 2, 128, 2), 112, 4, True),
     ((1, 64, 2, 128, 2), 112, 4, True),
+    ((0, 128, 1, 128, 2), 112, 4, True),
+    ((0, 128, 1, 128, 1), 112, 4, True),
+    ((0, 128, 1, 128, 0), 112, 4, False),
 ]
 
 
