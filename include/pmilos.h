@@ -267,7 +267,8 @@ void pm_graph_destroy(pm_graph *g);
  * first queued scan that fits; any scan when none is admitted), uploaded,
  * split into pixel chunks that run on several streams, the next chunk from
  * the admitted scan with the earliest deadline (its submission plus 1.5
- * times its estimated time alone on the GPU), and their results are copied
+ * times its estimated time alone on the GPU; once passed, now plus 1.5
+ * times the estimated time of its remaining work), and their results are copied
  * back into the caller's buffers.  A scan's results are the same bits as pm_invert / pm_invert_sl
  * give for it alone.  The caller keeps its buffers alive and unchanged
  * until pm_wait returns for the ticket.  One host thread may submit while

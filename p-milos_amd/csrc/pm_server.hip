@@ -2,7 +2,7 @@
 // pm_poll, pm_wait): scans of any size and wavelength sampling arrive at
 // any time and share the GPU.
 //
-// Host side only.  Two threads per server:
+// Host side only.  Three threads per server:
 //   dispatcher  splits every admitted scan into pixel chunks of about the
 //               same estimated work and keeps each compute stream busy with
 //               one chunk, always taking the next chunk from the admitted
@@ -12,23 +12,33 @@
 //               chunks it runs): a small scan that arrives behind a large
 //               one waits for one chunk per stream, not for the large scan,
 //               and a large scan is not held back indefinitely by a stream
-//               of smaller ones.  A chunk's Stokes samples are uploaded on
+//               of smaller ones.  A scan still running at its deadline gets
+//               a new one, the same way, from the work it has left: a late
+//               large scan does not hold every stream until it ends.  A chunk's Stokes samples are uploaded on
 //               its stream just before its kernel.  Chunks of different
 //               scans run at the same time: a persistent kernel's workgroups
 //               retire as its work queue drains and the next chunk's take
 //               their place.
-//   stager      admits queued scans while the device bytes of the admitted
+//   admitter    admits queued scans while the device bytes of the admitted
 //               ones stay within the HBM budget (the first queued scan that
-//               fits; any scan when none is admitted), and copies the
-//               results of every finished chunk back into the caller's
-//               buffers; a scan is done when its last chunk is back, which
+//               fits; any scan when none is admitted).
+//   copier      copies the results of finished chunks back into the
+//               caller's buffers, the chunk of the earliest deadline first
+//               (a copy into pageable memory can take far longer than the
+//               kernel; it delays later copies, not admissions or
+//               launches); a scan is done when its last chunk is back, which
 //               releases its budget.
-// Each compute stream has its own work-queue counters.  A pixel's result
+// Each compute stream has its own work-queue counters.
+// With PM_SERVER_TRACE set to a file name, the server writes one line per
+// finished scan there: its times of submission, admission, first chunk
+// launch, last kernel done and results back, in ms from the server's start.  A pixel's result
 // does not depend on which chunk, stream or workgroup computes it, so a scan
 // gives the same bits alone and in the mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -109,6 +119,7 @@ struct Req {
     int rc = PM_OK;
     State state = QUEUED;
     Clock::time_point t_submit, t_done;
+    Clock::time_point t_admit, t_launch, t_computed;   // for the trace
 };
 
 struct Chunk {
@@ -118,8 +129,15 @@ struct Chunk {
 };
 
 // A scan's deadline: its arrival plus SLACK times its estimated time alone
-// (at least MIN_SLACK_S).
+// (at least MIN_SLACK_S); a late scan's: now plus SLACK times the estimated
+// time of the work it has left.
 constexpr double SLACK = 1.5, MIN_SLACK_S = 0.02;
+
+static Clock::time_point deadline_after(Clock::time_point t, double work, double rate)
+{
+    return t + std::chrono::duration_cast<Clock::duration>(
+                   std::chrono::duration<double>(std::max(MIN_SLACK_S, SLACK * work / rate)));
+}
 
 struct Slot {
     hipStream_t s = nullptr;
@@ -161,9 +179,12 @@ struct pm_server {
     hipStream_t xfer = nullptr;
     std::vector<Slot> slots;
     DevPool pool;
-    std::thread stager, dispatcher;
+    std::thread admitter, copier, dispatcher;
+    FILE *trace = nullptr;               // PM_SERVER_TRACE
+    Clock::time_point t0;
 
-    void stager_loop();
+    void admit_loop();
+    void copy_loop();
     void dispatcher_loop();
     bool admit_one(std::unique_lock<std::mutex> &lk);
     bool copy_back(const Chunk &c);
@@ -201,6 +222,7 @@ bool pm_server::admit_one(std::unique_lock<std::mutex> &lk)
     r->d_acc = (int *)p; p += up(n * sizeof(int), ALIGN);
     if (r->fit) r->d_fit = (float *)p;
     r->state = READY;
+    r->t_admit = Clock::now();
     ready.push_back(r);
     cv.notify_all();
     return true;
@@ -217,10 +239,19 @@ void pm_server::complete(Req *r, int px)
     in_use -= r->bytes;
     r->t_done = Clock::now();
     r->state = DONE;
+    if (trace) {
+        auto ms = [&](Clock::time_point t) {
+            return t.time_since_epoch().count() == 0
+                       ? -1.0 : std::chrono::duration<double, std::milli>(t - t0).count();
+        };
+        std::fprintf(trace, "%lld %d %d %.3f %.3f %.3f %.3f %.3f\n", r->id, r->npix, r->L,
+                     ms(r->t_submit), ms(r->t_admit), ms(r->t_launch), ms(r->t_computed),
+                     ms(r->t_done));
+    }
     cv.notify_all();
 }
 
-// Copies a finished chunk's results into the caller's buffers (stager
+// Copies a finished chunk's results into the caller's buffers (copier
 // thread, without the lock); false on a HIP error.
 bool pm_server::copy_back(const Chunk &c)
 {
@@ -240,14 +271,29 @@ bool pm_server::copy_back(const Chunk &c)
     return ok;
 }
 
-void pm_server::stager_loop()
+void pm_server::admit_loop()
+{
+    (void)hipSetDevice(device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        if (!queued.empty() && admit_one(lk)) continue;
+        if (stop && idle()) break;
+        cv.wait(lk);
+    }
+    cv.notify_all();
+}
+
+void pm_server::copy_loop()
 {
     (void)hipSetDevice(device);
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
         if (!back.empty()) {
-            Chunk c = back.front();
-            back.pop_front();
+            auto it = std::min_element(back.begin(), back.end(), [](const Chunk &a, const Chunk &b) {
+                return a.r->deadline < b.r->deadline;
+            });
+            Chunk c = *it;
+            back.erase(it);
             const bool want = c.r->rc == PM_OK;
             lk.unlock();
             const bool ok = !want || copy_back(c);
@@ -257,7 +303,6 @@ void pm_server::stager_loop()
             complete(c.r, c.px);
             continue;
         }
-        if (!queued.empty() && admit_one(lk)) continue;
         if (stop && idle()) break;
         cv.wait(lk);
     }
@@ -271,6 +316,7 @@ bool pm_server::launch(Slot &sl, Req *r, std::unique_lock<std::mutex> &lk)
 {
     const int p0 = r->next_px, px = std::min(r->chunk, r->npix - p0);
     const size_t o4 = (size_t)p0 * 4 * r->L, cube = (size_t)px * 4 * r->L * sizeof(float);
+    if (p0 == 0) r->t_launch = Clock::now();
     r->next_px += px;
     if (r->next_px >= r->npix) ready.erase(std::find(ready.begin(), ready.end(), r));
     inflight++;
@@ -329,6 +375,7 @@ void pm_server::dispatcher_loop()
                 hipError_t e = hipEventQuery(sl.cur.ev);
                 if (e == hipErrorNotReady) { busy = true; continue; }
                 if (e != hipSuccess) sl.cur.r->rc = PM_EDEVICE;
+                sl.cur.r->t_computed = std::max(sl.cur.r->t_computed, now);
                 win_work += sl.cur.px * sl.cur.r->cost;
                 sl.spare.push_back(sl.cur.ev);
                 back.push_back(sl.cur);
@@ -341,6 +388,7 @@ void pm_server::dispatcher_loop()
             double best_w = 0;
             for (Req *r : ready) {
                 const double w = (r->npix - r->next_px) * r->cost;
+                if (r->deadline < now) r->deadline = deadline_after(now, w, rate);
                 if (!best || r->deadline < best->deadline ||
                     (r->deadline == best->deadline && w < best_w)) {
                     best = r;
@@ -380,6 +428,12 @@ extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
     }
     s->budget = o.hbm_budget;
     s->chunk_px = o.chunk_px;
+    s->t0 = Clock::now();
+    if (const char *f = std::getenv("PM_SERVER_TRACE")) {
+        s->trace = std::fopen(f, "w");
+        if (s->trace)
+            std::fprintf(s->trace, "# id npix L submit admit first_launch computed done (ms)\n");
+    }
     // per-pixel cost of the reference workload: 30 samples, 9 parameters,
     // a PSF of about 23 taps
     s->ref_cost = 30 * (400.0 + 10.0 * 9 * 23) + 10.0 * 729;
@@ -399,7 +453,8 @@ extern "C" int pm_server_create(const pm_server_opts *opts, pm_server **out)
         pm_server_destroy(s);
         return rc;
     }
-    s->stager = std::thread(&pm_server::stager_loop, s);
+    s->admitter = std::thread(&pm_server::admit_loop, s);
+    s->copier = std::thread(&pm_server::copy_loop, s);
     s->dispatcher = std::thread(&pm_server::dispatcher_loop, s);
     *out = s;
     return PM_OK;
@@ -413,7 +468,8 @@ extern "C" void pm_server_destroy(pm_server *s)
         s->stop = true;
         s->cv.notify_all();
     }
-    if (s->stager.joinable()) s->stager.join();
+    if (s->admitter.joinable()) s->admitter.join();
+    if (s->copier.joinable()) s->copier.join();
     if (s->dispatcher.joinable()) s->dispatcher.join();
     (void)hipSetDevice(s->device);
     for (auto &sl : s->slots) {
@@ -425,6 +481,7 @@ extern "C" void pm_server_destroy(pm_server *s)
     }
     if (s->xfer) (void)hipStreamDestroy(s->xfer);
     s->pool.release();
+    if (s->trace) std::fclose(s->trace);
     for (auto &kv : s->all) delete kv.second;
     delete s;
 }
@@ -454,10 +511,7 @@ extern "C" int pm_submit(pm_server *s, pm_ctx *ctx, const float *stokes, const f
     r->chunk = (int)std::max(1024.0, std::min(1e9, s->chunk_px * s->ref_cost / r->cost));
     r->t_submit = Clock::now();
     std::lock_guard<std::mutex> g(s->mu);
-    const double alone_s = npix * r->cost / s->rate;
-    r->deadline = r->t_submit + std::chrono::duration_cast<Clock::duration>(
-                                    std::chrono::duration<double>(
-                                        std::max(MIN_SLACK_S, SLACK * alone_s)));
+    r->deadline = deadline_after(r->t_submit, npix * r->cost, s->rate);
     r->id = s->next_id++;
     s->all[r->id] = r;
     if (npix == 0) {

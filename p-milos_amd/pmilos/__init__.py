@@ -7,6 +7,12 @@ control settings); Context.invert replaces the FITS pixel loop around lm_mils
 
 The HIP library is required: importing works without it, but every call
 raises if libpmilos_hip.so is missing (there is no CPU fallback).
+
+With torch in the same process, import torch before the first call here.
+torch ships its own HIP runtime; loaded after the library's (ROCm's
+libamdhip64.so.7), torch's finds no GPU.  Loaded first, both work on one
+device address space (torch-allocated buffers go straight to the *_device
+and *_ptrs calls).
 """
 import ctypes as C
 import os
@@ -275,15 +281,38 @@ class Server:
         self.h = h
         self._pending = {}
 
-    def submit(self, ctx, stokes, slight=None, want_fit=False):
-        """Queues a scan; returns a ticket.  The result buffers are allocated
-        here and kept with the ticket."""
+    @staticmethod
+    def result_buffers(n, L=0, want_fit=False, alloc=np.empty):
+        """A result set for a scan of n pixels (the `out` of submit()):
+        model [n, 11] f64, chi2 f32, iters and accepted i32, and with
+        want_fit the fitted profiles [n, 4, L] f32.  alloc(shape, dtype)
+        makes each array (e.g. one that returns pinned host memory)."""
+        return dict(model=alloc((n, NPAR), np.float64), chi2=alloc((n,), np.float32),
+                    iters=alloc((n,), np.int32), accepted=alloc((n,), np.int32),
+                    fit=alloc((n, 4, L), np.float32) if want_fit else None)
+
+    def submit(self, ctx, stokes, slight=None, want_fit=False, out=None):
+        """Queues a scan; returns a ticket.  The results go into `out` (a
+        result_buffers() set of the scan's size; pinned host memory copies
+        back fastest) or into buffers allocated here; either is kept with
+        the ticket and returned by wait()."""
         st = np.ascontiguousarray(stokes, dtype=np.float32)
         n = st.shape[0]
         sl = None if slight is None else np.ascontiguousarray(slight, dtype=np.float32)
-        out = dict(model=np.zeros((n, NPAR)), chi2=np.zeros(n, dtype=np.float32),
-                   iters=np.zeros(n, dtype=np.int32), accepted=np.zeros(n, dtype=np.int32),
-                   fit=np.zeros((n, 4, ctx.L), dtype=np.float32) if want_fit else None)
+        if out is None:
+            out = self.result_buffers(n, ctx.L, want_fit, np.zeros)
+        else:
+            want = self.result_buffers(n, ctx.L, want_fit, lambda sh, dt: (sh, dt))
+            for k, v in want.items():
+                a = out.get(k)
+                if v is None:
+                    continue
+                if (not isinstance(a, np.ndarray) or a.shape != v[0] or a.dtype != v[1]
+                        or not a.flags.c_contiguous or not a.flags.writeable):
+                    raise ValueError("out[%r]: need a writable C-contiguous %s array of shape %s"
+                                     % (k, np.dtype(v[1]).name, v[0]))
+            if not want_fit:
+                out = dict(out, fit=None)
         t = C.c_longlong(0)
         _check(lib().pm_submit(self.h, ctx.h, st.ctypes.data, _ptr(sl), n,
                                out["model"].ctypes.data, out["chi2"].ctypes.data,

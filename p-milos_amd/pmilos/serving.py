@@ -12,7 +12,10 @@ k is synthesised from seed 500 + k (models and noise, made on the GPU) for
 the first few scans of each (size, sampling) kind -- four of each kind up to
 256^2 pixels, two of the larger ones -- and later scans of a kind reuse
 those cubes in turn (a 2048^2 x 112 cube is 7.5 GB of host memory).  The
-cubes are in pinned host memory; the results land in pageable arrays.
+cubes are in pinned host memory, and so are the results: each scan's
+result set comes from a pool of pinned sets of its size, returned to the
+pool once the scan has been collected (a serving client's reuse of its
+buffers; fresh pageable result arrays cost page faults on every copy).
 
     python -m pmilos.serving [--scans N] [--util U ...] [--precision mixed|fp64]
     python -m pmilos.serving --per-sampling 25 --small 50   (the benchmark's mix)
@@ -121,15 +124,44 @@ def calibrate(ctxs, made):
     return rate
 
 
+def _pinned(shape, dtype):
+    import torch
+    t = torch.empty(shape, dtype=getattr(torch, np.dtype(dtype).name), pin_memory=True)
+    return t.numpy()
+
+
+class ResultPool:
+    """Pinned result sets by scan size, reused once collected; `ahead` sets
+    of every size in `sizes` are made up front."""
+
+    def __init__(self, sizes=(), ahead=2):
+        self.free = {}
+        self.lock = threading.Lock()
+        for n in set(sizes):
+            self.free[n] = [pmilos.Server.result_buffers(n, alloc=_pinned) for _ in range(ahead)]
+
+    def take(self, n):
+        with self.lock:
+            lst = self.free.get(n)
+            if lst:
+                return lst.pop()
+        return pmilos.Server.result_buffers(n, alloc=_pinned)
+
+    def give(self, n, out):
+        with self.lock:
+            self.free.setdefault(n, []).append(out)
+
+
 def run(server, ctxs, cubes, kinds, rate_hz, seed=50, progress=False):
     """Submits len(kinds) scans (scan k: cubes[k]) at Poisson arrival times;
     returns per-scan (kind, npix, latency) and the wall time from first
-    arrival to last completion.  progress: a line on stderr every 50
-    completed scans."""
+    arrival to last completion.  Results go into pinned sets from a
+    ResultPool.  progress: a line on stderr every 50 completed scans."""
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / rate_hz, len(kinds))
     tickets = []
     done = threading.Event()
+    pool = ResultPool([k[0] * k[0] for k in kinds])
 
     def submitter():
         t_next = time.perf_counter()
@@ -139,7 +171,7 @@ def run(server, ctxs, cubes, kinds, rate_hz, seed=50, progress=False):
             if d > 0:
                 time.sleep(d)
             size, L = k
-            tickets.append((k, server.submit(ctxs[L], cube)))
+            tickets.append((k, server.submit(ctxs[L], cube, out=pool.take(size * size))))
         done.set()
 
     t0 = time.perf_counter()
@@ -150,7 +182,8 @@ def run(server, ctxs, cubes, kinds, rate_hz, seed=50, progress=False):
     while True:
         if i < len(tickets):
             k, t = tickets[i]
-            _, lat = server.wait(t)
+            out, lat = server.wait(t)
+            pool.give(k[0] * k[0], out)
             res.append((k, k[0] * k[0], lat))
             i += 1
             if progress and i % 50 == 0:

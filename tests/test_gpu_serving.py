@@ -6,6 +6,7 @@ import sys
 
 import numpy as np
 import pytest
+import torch  # noqa: F401  (its HIP runtime loads before the library's)
 
 import oracle as O
 import pmilos
@@ -67,6 +68,32 @@ def test_mix_equals_alone():
             if out["fit"] is None:
                 ref["fit"] = None
             _same(out, ref)
+    finally:
+        srv.close()
+
+
+def test_results_into_caller_buffers():
+    """submit(out=...): results land in the caller's result set, pinned or
+    pageable, reused for a second scan; a set of the wrong size is
+    refused before anything is queued."""
+    say("case serving caller buffers")
+    from pmilos.serving import _pinned
+    ctx = _ctx(30, pmilos.PM_MIXED)
+    n = 3000
+    cubes = [_cube(ctx, n, 300 + i) for i in range(2)]
+    alone = [ctx.invert(q) for q in cubes]
+    srv = pmilos.Server(streams=2, chunk_px=1024)
+    try:
+        for alloc in (_pinned, np.empty):
+            out = pmilos.Server.result_buffers(n, alloc=alloc)
+            for k in ("model", "chi2"):
+                out[k][...] = np.nan
+            for q, ref in zip(cubes, alone):
+                got, _ = srv.wait(srv.submit(ctx, q, out=out))
+                assert got["model"] is out["model"]
+                _same(got, ref)
+        with pytest.raises(ValueError):
+            srv.submit(ctx, cubes[0], out=pmilos.Server.result_buffers(n - 1))
     finally:
         srv.close()
 
