@@ -11,8 +11,9 @@ as float32.  Inversion: 9 free parameters, <= 50 LM iterations, the run/
 guess.mod initial model plus classical estimates per pixel.
 
 One step = one inversion pass over the whole cube, inputs resident in HBM.
-Beside it: one pass of each form from the cube in pinned host memory to the
-results in host memory (PCIe copies included), and one pass of each form
+Beside it: --steps passes of each form through pm_invert from the cube in
+pinned host memory to the results in pinned host memory (PCIe copies
+included, overlapped with the kernels by the library), and one pass of each form
 with the logclambda damping rule of run/pmilos.mtrol (logclambda = 1).
 The same field at noise sigma 1e-4 and 3e-3 Ic gives one more pass of each
 form (the mixed form's error against the FP64 form and the change in
@@ -378,26 +379,31 @@ def main():
                     fallbacks=c.fallback_count(), models=out_m.cpu().numpy(), iters=iters)
 
     def host_to_host(c):
-        """Passes from the float32 cube in pinned host memory to the results
-        in pinned host memory (upload, inversion, copy-back on one stream),
-        wall clock averaged over --steps passes; and the copies alone."""
-        h_in = torch.empty(stokes.shape, dtype=torch.float32, pin_memory=True)
-        h_in.copy_(stokes)
+        """Passes of pm_invert from the float32 cube in pinned host memory to
+        the results in pinned host memory (the library's host-buffer path:
+        the pixels in chunks over its streams, each chunk's upload and
+        copy-back overlapping other chunks' kernels), wall clock averaged
+        over --steps passes; and the upload and copy-back alone, back to
+        back on one stream."""
+        def pinned(shape, dtype):
+            return torch.empty(shape, dtype=getattr(torch, np.dtype(dtype).name),
+                               pin_memory=True).numpy()
+
+        h_in = pinned(tuple(stokes.shape), np.float32)
+        h_in[...] = stokes.cpu().numpy()
+        out = pmilos.result_buffers(npix, alloc=pinned)
         d_in = torch.empty_like(stokes)
         outs = (out_m, out_c, out_i, out_a)
-        h_out = [torch.empty(o.shape, dtype=o.dtype, pin_memory=True) for o in outs]
+        t_in = torch.from_numpy(h_in)
+        t_out = [torch.from_numpy(out[k]) for k in ("model", "chi2", "iters", "accepted")]
 
         def copies():
-            d_in.copy_(h_in, non_blocking=True)
-            for h, o in zip(h_out, outs):
+            d_in.copy_(t_in, non_blocking=True)
+            for h, o in zip(t_out, outs):
                 h.copy_(o, non_blocking=True)
 
         def whole():
-            d_in.copy_(h_in, non_blocking=True)
-            c.invert_ptrs(d_in.data_ptr(), npix, out_m.data_ptr(), out_c.data_ptr(),
-                          out_i.data_ptr(), out_a.data_ptr(), 0, sptr)
-            for h, o in zip(h_out, outs):
-                h.copy_(o, non_blocking=True)
+            c.invert(h_in, out=out)
 
         res = {}
         for name, fn in (("copies_only", copies), ("whole", whole)):
@@ -408,7 +414,7 @@ def main():
                 fn()
             torch.cuda.synchronize()
             res[name] = (time.perf_counter() - t) / args.steps
-        del d_in, h_in, h_out
+        del d_in, h_in, out, t_in, t_out
         return {"pixels_per_s": round(npix / res["whole"], 1),
                 "ms": round(res["whole"] * 1e3, 3),
                 "copies_only_ms": round(res["copies_only"] * 1e3, 3),

@@ -131,7 +131,14 @@ int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectr
  * (last accepted chi2); iters [npix] int (LM trips, or the pixel index when
  * lm_mils returns -1 for nfree == 0, as milos.c:997 leaves it); accepted
  * nullable [npix] int (chi2-decreasing trips); fit nullable [npix][4][L] float
- * (the last trial synthesis, milosUtils.c:569).  Host pointers, synchronous. */
+ * (the last trial synthesis, milosUtils.c:569).  Host pointers, synchronous:
+ * from 2^18 pixels on, the pixels go in three chunks (1/16, 14/16, 1/16) on
+ * three streams of the context, each chunk's upload and copy-back
+ * overlapping the other chunks' kernels (pinned host buffers copy
+ * fastest), with the same results as one pm_invert_device call.  The device buffers of the host-pointer calls (pm_synth,
+ * pm_invert, pm_invert_sl, pm_classical, pm_ctx_autotune_host) belong to
+ * the context: made on first use, grown when a call needs more, kept until
+ * pm_ctx_destroy. */
 int pm_invert(pm_ctx *ctx, const float *stokes, int npix, double *models,
               float *chi2, int *iters, int *accepted, float *fit);
 

@@ -22,6 +22,18 @@
 
 // ============================================================ host side
 
+// The host-pointer inversion runs its pixels in up to PM_HOST_CHUNKS chunks,
+// one stream each (upload, kernel and copy-back of a chunk in its stream's
+// order, so that copies overlap other chunks' kernels): from
+// PM_HOST_SPLIT_MIN pixels on, 1/16, 14/16 and 1/16 of them.  The small
+// first chunk's kernel starts after a short upload and runs while the big
+// chunk uploads; the small last one runs while the big one's results come
+// back.  Equal chunks do worse: every chunk boundary leaves part of the GPU
+// idle while a persistent kernel drains (1024^2 pixels x 30 samples, FP64,
+// pinned buffers: one chunk 161.6 ms, eight equal ones 162.3 ms, 1/16 +
+// 15/16 153.1 ms, 1/16 + 14/16 + 1/16 154.7 ms).
+constexpr int PM_HOST_STREAMS = 3, PM_HOST_CHUNKS = 3, PM_HOST_SPLIT_MIN = 1 << 18;
+
 struct pm_ctx {
     PmConst h;
     PmConst *d = nullptr;
@@ -42,6 +54,16 @@ struct pm_ctx {
     hipEvent_t done = nullptr;
     bool launched = false;
     void *scratch = nullptr;   // this device's scratch (pm_invert_device_noalloc)
+    // the host-pointer calls' device buffers and streams, made on first use
+    // and kept (grown, never shrunk) until pm_ctx_destroy
+    struct {
+        hipStream_t s[PM_HOST_STREAMS] = {};      // chunk k on s[k]
+        char *ws = nullptr;
+        size_t ws_bytes = 0;
+        int *counters = nullptr;            // a set per chunk
+        unsigned long long *fallbacks = nullptr;   // one per chunk
+        hipEvent_t cleared = nullptr;       // every chunk's counters zeroed
+    } host;
 };
 
 // Orders a launch that uses the context's counters after the previous one
@@ -203,6 +225,15 @@ extern "C" void pm_ctx_destroy(pm_ctx *ctx)
     if (ctx->counters) (void)hipFree(ctx->counters);
     if (ctx->fallbacks) (void)hipFree(ctx->fallbacks);
     if (ctx->done) (void)hipEventDestroy(ctx->done);
+    for (auto st : ctx->host.s)
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    if (ctx->host.ws) (void)hipFree(ctx->host.ws);
+    if (ctx->host.counters) (void)hipFree(ctx->host.counters);
+    if (ctx->host.fallbacks) (void)hipFree(ctx->host.fallbacks);
+    if (ctx->host.cleared) (void)hipEventDestroy(ctx->host.cleared);
     delete ctx;
 }
 
@@ -213,7 +244,7 @@ extern "C" int pm_ctx_nterms(const pm_ctx *ctx) { return ctx ? ctx->h.NT : PM_EI
 // share: the fraction of the GPU's resident workgroups the launch may take
 // (1 = all of them).
 template <typename T, int W, int NLPL, int NT, bool GEN, int BLK>
-static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st, double share)
+static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st, double share, bool reset)
 {
     using LD = Lds<T, W, NLPL, NT, GEN, BLK>;
     auto kern = pm_invert_kernel<T, W, NLPL, NT, GEN, BLK>;
@@ -224,8 +255,10 @@ static int launch_invert(pm_ctx *ctx, const InvArgs &a, hipStream_t st, double s
     long long grid = (long long)(per_cu * ctx->ncu * (share > 0 && share < 1 ? share : 1.0) + 0.5);
     if (grid > need) grid = need;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, a.counters, a.fallbacks);
-    PM_HIP(hipGetLastError());
+    if (reset) {
+        hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, a.counters, a.fallbacks);
+        PM_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(LD::BLOCK), 0, st, a);
     PM_HIP(hipGetLastError());
     return PM_OK;
@@ -256,7 +289,7 @@ static int launch_synth(pm_ctx *ctx, const double *m, int n, double *sp, double 
 // FP32); the general form (NTERMS 10/11, macroturbulence, stray light) is
 // the 10- or 11-row kernel.  The synthesis kernel
 // follows the selected variant's lane mapping.
-typedef int (*InvLaunch)(pm_ctx *, const InvArgs &, hipStream_t, double);
+typedef int (*InvLaunch)(pm_ctx *, const InvArgs &, hipStream_t, double, bool);
 typedef int (*SynLaunch)(pm_ctx *, const double *, int, double *, double *, hipStream_t);
 struct InvVariant {
     int W, NLPL, BLK;
@@ -393,7 +426,7 @@ extern "C" int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, doubl
 int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, int npix,
                      double *d_models, float *d_chi2, int *d_iters, int *d_accepted,
                      float *d_fit, int *d_counters, unsigned long long *d_fallbacks,
-                     hipStream_t st, double share)
+                     hipStream_t st, double share, bool reset)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
@@ -412,7 +445,7 @@ int pm_invert_launch(pm_ctx *ctx, const float *d_stokes, const float *d_slight, 
     a.fallbacks = d_fallbacks;
     a.slight = d_slight;
     const bool gen = needs_general(ctx, d_slight != nullptr);
-    return selected(ctx, gen, d_slight != nullptr).fn(ctx, a, st, share);
+    return selected(ctx, gen, d_slight != nullptr).fn(ctx, a, st, share, reset);
 }
 
 int pm_ctx_nlambda(const pm_ctx *ctx) { return ctx->h.L; }
@@ -429,6 +462,13 @@ double pm_ctx_cost(const pm_ctx *ctx)
 }
 int pm_ctx_device(const pm_ctx *ctx) { return ctx->device; }
 size_t pm_counters_bytes() { return sizeof(int) * PM_NSLICE * PM_CSTRIDE; }
+
+int pm_reset_launch(int *d_counters, unsigned long long *d_fallbacks, hipStream_t st)
+{
+    hipLaunchKernelGGL(pm_reset_counters, dim3(1), dim3(64), 0, st, d_counters, d_fallbacks);
+    PM_HIP(hipGetLastError());
+    return PM_OK;
+}
 
 extern "C" int pm_invert_sl_device(pm_ctx *ctx, const float *d_stokes, const float *d_slight,
                                    int npix, double *d_models, float *d_chi2, int *d_iters,
@@ -536,9 +576,9 @@ extern "C" int pm_ctx_autotune(pm_ctx *ctx, const float *d_stokes, const float *
     float best_ms = 0;
     for (int k = 0; k < nv && rc == PM_OK; k++) {
         float ms = 0;
-        rc = v[k].fn(ctx, a, st, 1.0);                       // first run: not timed
+        rc = v[k].fn(ctx, a, st, 1.0, true);                 // first run: not timed
         if (rc == PM_OK && hipEventRecord(e0, st) != hipSuccess) rc = PM_EDEVICE;
-        for (int rep = 0; rep < 2 && rc == PM_OK; rep++) rc = v[k].fn(ctx, a, st, 1.0);
+        for (int rep = 0; rep < 2 && rc == PM_OK; rep++) rc = v[k].fn(ctx, a, st, 1.0, true);
         if (rc == PM_OK && (hipEventRecord(e1, st) != hipSuccess ||
                             hipEventSynchronize(e1) != hipSuccess ||
                             hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
@@ -720,58 +760,156 @@ extern "C" int pm_graph_launch(pm_graph *g, void *stream)
 // ------------------------------------------------------------ host-pointer wrappers
 
 namespace {
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    int alloc(size_t n) { return hipMalloc(&p, n ? n : 1) == hipSuccess ? PM_OK : PM_ENOMEM; }
+// The host calls' streams, counters and per-chunk fallback counters, made
+// on the first host call of a context.
+int host_setup(pm_ctx *ctx)
+{
+    auto &h = ctx->host;
+    if (h.counters) return PM_OK;
+    for (auto &st : h.s)
+        if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+            return PM_EDEVICE;
+    if (hipMalloc(&h.fallbacks, sizeof(unsigned long long) * PM_HOST_CHUNKS) != hipSuccess)
+        return PM_ENOMEM;
+    if (!h.cleared && hipEventCreateWithFlags(&h.cleared, hipEventDisableTiming) != hipSuccess)
+        return PM_EDEVICE;
+    if (hipMalloc(&h.counters, pm_counters_bytes() * PM_HOST_CHUNKS) != hipSuccess) {
+        (void)hipFree(h.fallbacks);
+        h.fallbacks = nullptr;
+        return PM_ENOMEM;
+    }
+    return PM_OK;
+}
+
+// The host calls' device buffer, at least `bytes` (the old one is freed
+// when it grows: the calls that use it are synchronous, so none is in
+// flight).
+int host_ws(pm_ctx *ctx, size_t bytes, char **out)
+{
+    auto &h = ctx->host;
+    if (h.ws_bytes < bytes) {
+        if (h.ws) (void)hipFree(h.ws);
+        h.ws = nullptr;
+        h.ws_bytes = 0;
+        if (hipMalloc(&h.ws, bytes) != hipSuccess) return PM_ENOMEM;
+        h.ws_bytes = bytes;
+    }
+    *out = h.ws;
+    return PM_OK;
+}
+
+// Waits for every host stream of the context (on every exit of a host
+// call: their buffers are reused by the next one).
+struct HostSync {
+    pm_ctx *ctx;
+    ~HostSync()
+    {
+        for (auto st : ctx->host.s)
+            if (st) (void)hipStreamSynchronize(st);
+    }
 };
+
+size_t al(size_t n) { return (n + 255) / 256 * 256; }
 }  // namespace
 
 extern "C" int pm_synth(pm_ctx *ctx, const double *models, int n, double *spectra, double *rf)
 {
     if (!ctx || n < 0 || (n > 0 && (!models || !spectra))) return PM_EINVAL;
     if (n == 0) return PM_OK;
-    const size_t L = ctx->h.L, NT = ctx->h.NT;
-    DevBuf dm, ds, dr;
-    if (dm.alloc(sizeof(double) * 11 * n) || ds.alloc(sizeof(double) * 4 * L * n) ||
-        (rf && dr.alloc(sizeof(double) * 4 * NT * L * n)))
-        return PM_ENOMEM;
-    PM_HIP(hipMemcpy(dm.p, models, sizeof(double) * 11 * n, hipMemcpyHostToDevice));
-    int rc = pm_synth_device(ctx, (double *)dm.p, n, (double *)ds.p, rf ? (double *)dr.p : nullptr,
-                             nullptr);
+    const size_t L = ctx->h.L, NT = ctx->h.NT, bm = sizeof(double) * 11 * n,
+                 bs = sizeof(double) * 4 * L * n, br = rf ? sizeof(double) * 4 * NT * L * n : 0;
+    int rc = host_setup(ctx);
     if (rc) return rc;
-    PM_HIP(hipDeviceSynchronize());
-    PM_HIP(hipMemcpy(spectra, ds.p, sizeof(double) * 4 * L * n, hipMemcpyDeviceToHost));
-    if (rf) PM_HIP(hipMemcpy(rf, dr.p, sizeof(double) * 4 * NT * L * n, hipMemcpyDeviceToHost));
+    HostSync sync{ctx};
+    char *w = nullptr;
+    if ((rc = host_ws(ctx, al(bm) + al(bs) + al(br), &w))) return rc;
+    double *dm = (double *)w, *ds = (double *)(w + al(bm)), *dr = (double *)(w + al(bm) + al(bs));
+    hipStream_t st = ctx->host.s[0];
+    PM_HIP(hipMemcpyAsync(dm, models, bm, hipMemcpyHostToDevice, st));
+    if ((rc = pm_synth_device(ctx, dm, n, ds, rf ? dr : nullptr, st))) return rc;
+    PM_HIP(hipMemcpyAsync(spectra, ds, bs, hipMemcpyDeviceToHost, st));
+    if (rf) PM_HIP(hipMemcpyAsync(rf, dr, br, hipMemcpyDeviceToHost, st));
+    PM_HIP(hipStreamSynchronize(st));
     return PM_OK;
 }
 
+// Host buffers: the pixels go in chunks (PM_HOST_CHUNKS), chunk k on the
+// context's host stream k in its own slice of the device buffers: upload,
+// kernel, copy-back.  Every chunk has its own work-queue counters and
+// fallback counter, all zeroed before the first upload: a counter reset
+// queued behind another chunk would wait for a CU that the other stream's
+// persistent kernel holds, and hold back its own chunk's kernel.  Every
+// chunk's kernel is queued before any copy-back, since a copy into pageable
+// memory returns only when it is done, i.e. after its chunk's kernel.  A
+// pixel's results do not depend on its chunk, so the bits are those of one
+// pm_invert_sl_device over all pixels.
 extern "C" int pm_invert_sl(pm_ctx *ctx, const float *stokes, const float *slight, int npix,
                             double *models, float *chi2, int *iters, int *accepted, float *fit)
 {
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
     if (!stokes || !models || !chi2 || !iters) return PM_EINVAL;
-    const size_t L = ctx->h.L, n = npix;
-    DevBuf ds, dl, dm, dc, di, da, df;
-    if (ds.alloc(sizeof(float) * 4 * L * n) || dm.alloc(sizeof(double) * 11 * n) ||
-        dc.alloc(sizeof(float) * n) || di.alloc(sizeof(int) * n) ||
-        (slight && dl.alloc(sizeof(float) * 4 * L * n)) ||
-        (accepted && da.alloc(sizeof(int) * n)) || (fit && df.alloc(sizeof(float) * 4 * L * n)))
-        return PM_ENOMEM;
-    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
-    if (slight) PM_HIP(hipMemcpy(dl.p, slight, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
-    int rc = pm_invert_sl_device(ctx, (const float *)ds.p, slight ? (const float *)dl.p : nullptr,
-                                 npix, (double *)dm.p, (float *)dc.p, (int *)di.p,
-                                 accepted ? (int *)da.p : nullptr, fit ? (float *)df.p : nullptr,
-                                 nullptr);
+    const size_t L = ctx->h.L, n = npix, bst = sizeof(float) * 4 * L * n;
+    int bound[PM_HOST_CHUNKS + 1] = {0, npix, npix, npix};
+    int nch = 1;
+    if (npix >= PM_HOST_SPLIT_MIN) {
+        const int small = npix / 16;
+        bound[1] = small;
+        bound[2] = npix - small;
+        nch = 3;
+    }
+    const size_t off_sl = al(bst), off_m = off_sl + (slight ? al(bst) : 0),
+                 off_c = off_m + al(sizeof(double) * 11 * n), off_i = off_c + al(sizeof(float) * n),
+                 off_a = off_i + al(sizeof(int) * n), off_f = off_a + (accepted ? al(sizeof(int) * n) : 0),
+                 total = off_f + (fit ? al(bst) : 0);
+    int rc = host_setup(ctx);
     if (rc) return rc;
-    PM_HIP(hipDeviceSynchronize());
-    PM_HIP(hipMemcpy(models, dm.p, sizeof(double) * 11 * n, hipMemcpyDeviceToHost));
-    PM_HIP(hipMemcpy(chi2, dc.p, sizeof(float) * n, hipMemcpyDeviceToHost));
-    PM_HIP(hipMemcpy(iters, di.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (accepted) PM_HIP(hipMemcpy(accepted, da.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (fit) PM_HIP(hipMemcpy(fit, df.p, sizeof(float) * 4 * L * n, hipMemcpyDeviceToHost));
+    HostSync sync{ctx};
+    char *w = nullptr;
+    if ((rc = host_ws(ctx, total, &w))) return rc;
+    float *d_st = (float *)w, *d_sl = slight ? (float *)(w + off_sl) : nullptr;
+    double *d_m = (double *)(w + off_m);
+    float *d_c = (float *)(w + off_c), *d_f = fit ? (float *)(w + off_f) : nullptr;
+    int *d_i = (int *)(w + off_i), *d_a = accepted ? (int *)(w + off_a) : nullptr;
+    // the fallback total is written where pm_last_fallback_count reads it:
+    // no device-pointer call may still be using it
+    if (ctx->launched) PM_HIP(hipEventSynchronize(ctx->done));
+    auto &h = ctx->host;
+    const size_t cset = pm_counters_bytes() / sizeof(int);
+    for (int k = 0; k < nch; k++)
+        if ((rc = pm_reset_launch(h.counters + k * cset, h.fallbacks + k, h.s[0]))) return rc;
+    PM_HIP(hipEventRecord(h.cleared, h.s[0]));
+    for (int k = 1; k < nch; k++) PM_HIP(hipStreamWaitEvent(h.s[k], h.cleared, 0));
+    for (int k = 0; k < nch; k++) {
+        hipStream_t st = h.s[k];
+        const size_t p0 = bound[k], m = bound[k + 1] - bound[k], q0 = p0 * 4 * L;
+        const size_t sb = sizeof(float) * 4 * L * m;
+        PM_HIP(hipMemcpyAsync(d_st + q0, stokes + q0, sb, hipMemcpyHostToDevice, st));
+        if (slight) PM_HIP(hipMemcpyAsync(d_sl + q0, slight + q0, sb, hipMemcpyHostToDevice, st));
+        rc = pm_invert_launch(ctx, d_st + q0, slight ? d_sl + q0 : nullptr, (int)m, d_m + p0 * 11,
+                              d_c + p0, d_i + p0, d_a ? d_a + p0 : nullptr, d_f ? d_f + q0 : nullptr,
+                              h.counters + k * cset, h.fallbacks + k, st, 1.0, false);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nch; k++) {
+        hipStream_t st = h.s[k];
+        const size_t p0 = bound[k], m = bound[k + 1] - bound[k], q0 = p0 * 4 * L;
+        PM_HIP(hipMemcpyAsync(models + p0 * 11, d_m + p0 * 11, sizeof(double) * 11 * m,
+                              hipMemcpyDeviceToHost, st));
+        PM_HIP(hipMemcpyAsync(chi2 + p0, d_c + p0, sizeof(float) * m, hipMemcpyDeviceToHost, st));
+        PM_HIP(hipMemcpyAsync(iters + p0, d_i + p0, sizeof(int) * m, hipMemcpyDeviceToHost, st));
+        if (accepted)
+            PM_HIP(hipMemcpyAsync(accepted + p0, d_a + p0, sizeof(int) * m, hipMemcpyDeviceToHost,
+                                  st));
+        if (fit)
+            PM_HIP(hipMemcpyAsync(fit + q0, d_f + q0, sizeof(float) * 4 * L * m,
+                                  hipMemcpyDeviceToHost, st));
+    }
+    for (int k = 0; k < nch; k++) PM_HIP(hipStreamSynchronize(h.s[k]));
+    unsigned long long fb[PM_HOST_CHUNKS] = {}, tot = 0;
+    PM_HIP(hipMemcpy(fb, h.fallbacks, sizeof(unsigned long long) * nch, hipMemcpyDeviceToHost));
+    for (int k = 0; k < nch; k++) tot += fb[k];
+    PM_HIP(hipMemcpy(ctx->fallbacks, &tot, sizeof(tot), hipMemcpyHostToDevice));
     return PM_OK;
 }
 
@@ -786,14 +924,19 @@ extern "C" int pm_classical(pm_ctx *ctx, const float *stokes, int npix, double *
     if (!ctx || npix < 0) return PM_EINVAL;
     if (npix == 0) return PM_OK;
     if (!stokes || !models) return PM_EINVAL;
-    const size_t L = ctx->h.L, n = npix;
-    DevBuf ds, dm;
-    if (ds.alloc(sizeof(float) * 4 * L * n) || dm.alloc(sizeof(double) * 11 * n)) return PM_ENOMEM;
-    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * 4 * L * n, hipMemcpyHostToDevice));
-    int rc = pm_classical_device(ctx, (const float *)ds.p, npix, (double *)dm.p, nullptr);
+    const size_t L = ctx->h.L, n = npix, bs = sizeof(float) * 4 * L * n,
+                 bm = sizeof(double) * 11 * n;
+    int rc = host_setup(ctx);
     if (rc) return rc;
-    PM_HIP(hipDeviceSynchronize());
-    PM_HIP(hipMemcpy(models, dm.p, sizeof(double) * 11 * n, hipMemcpyDeviceToHost));
+    HostSync sync{ctx};
+    char *w = nullptr;
+    if ((rc = host_ws(ctx, al(bs) + al(bm), &w))) return rc;
+    hipStream_t st = ctx->host.s[0];
+    PM_HIP(hipMemcpyAsync(w, stokes, bs, hipMemcpyHostToDevice, st));
+    if ((rc = pm_classical_device(ctx, (const float *)w, npix, (double *)(w + al(bs)), st)))
+        return rc;
+    PM_HIP(hipMemcpyAsync(models, w + al(bs), bm, hipMemcpyDeviceToHost, st));
+    PM_HIP(hipStreamSynchronize(st));
     return PM_OK;
 }
 
@@ -801,11 +944,15 @@ extern "C" int pm_ctx_autotune_host(pm_ctx *ctx, const float *stokes, const floa
                                     int npix, int chosen[3])
 {
     if (!ctx || !stokes || npix < 1) return PM_EINVAL;
-    const size_t n = (size_t)ctx->h.L * 4 * npix;
-    DevBuf ds, dl;
-    if (ds.alloc(sizeof(float) * n) || (slight && dl.alloc(sizeof(float) * n))) return PM_ENOMEM;
-    PM_HIP(hipMemcpy(ds.p, stokes, sizeof(float) * n, hipMemcpyHostToDevice));
-    if (slight) PM_HIP(hipMemcpy(dl.p, slight, sizeof(float) * n, hipMemcpyHostToDevice));
-    return pm_ctx_autotune(ctx, (const float *)ds.p, slight ? (const float *)dl.p : nullptr, npix,
-                           nullptr, chosen);
+    const size_t bs = sizeof(float) * (size_t)ctx->h.L * 4 * npix;
+    int rc = host_setup(ctx);
+    if (rc) return rc;
+    HostSync sync{ctx};
+    char *w = nullptr;
+    if ((rc = host_ws(ctx, al(bs) * (slight ? 2 : 1), &w))) return rc;
+    hipStream_t st = ctx->host.s[0];
+    PM_HIP(hipMemcpyAsync(w, stokes, bs, hipMemcpyHostToDevice, st));
+    if (slight) PM_HIP(hipMemcpyAsync(w + al(bs), slight, bs, hipMemcpyHostToDevice, st));
+    return pm_ctx_autotune(ctx, (const float *)w, slight ? (const float *)(w + al(bs)) : nullptr,
+                           npix, st, chosen);
 }

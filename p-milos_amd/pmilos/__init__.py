@@ -187,23 +187,21 @@ class Context:
         _check(lib().pm_synth(self.h, m.ctypes.data, n, spec.ctypes.data, _ptr(d)))
         return (spec, d) if rf else spec
 
-    def invert(self, stokes, want_fit=False, slight=None):
-        """slight: optional per-pixel stray light [n][4][L] (else the
-        context's profile, if any)."""
+    def invert(self, stokes, want_fit=False, slight=None, out=None):
+        """pm_invert_sl on host arrays.  slight: optional per-pixel stray
+        light [n][4][L] (else the context's profile, if any).  out: a
+        result_buffers() set to write into (pinned host memory copies back
+        fastest), else one is allocated."""
         st = np.ascontiguousarray(stokes, dtype=np.float32)
         n = st.shape[0]
         sl = None if slight is None else np.ascontiguousarray(slight, dtype=np.float32)
         if sl is not None and sl.shape != st.shape:
             raise PmError("slight must have the shape of stokes")
-        models = np.zeros((n, NPAR))
-        chi2 = np.zeros(n, dtype=np.float32)
-        iters = np.zeros(n, dtype=np.int32)
-        acc = np.zeros(n, dtype=np.int32)
-        fit = np.zeros((n, 4, self.L), dtype=np.float32) if want_fit else None
-        _check(lib().pm_invert_sl(self.h, st.ctypes.data, _ptr(sl), n, models.ctypes.data,
-                                  chi2.ctypes.data, iters.ctypes.data, acc.ctypes.data,
-                                  _ptr(fit)))
-        return dict(model=models, chi2=chi2, iters=iters, accepted=acc, fit=fit)
+        out = _result_set(n, self.L, want_fit, out)
+        _check(lib().pm_invert_sl(self.h, st.ctypes.data, _ptr(sl), n, out["model"].ctypes.data,
+                                  out["chi2"].ctypes.data, out["iters"].ctypes.data,
+                                  out["accepted"].ctypes.data, _ptr(out["fit"])))
+        return out
 
     def invert_ptrs(self, d_stokes, npix, d_models, d_chi2, d_iters, d_accepted=0, d_fit=0,
                     stream=0, d_slight=0):
@@ -270,6 +268,33 @@ class Context:
         return v.value
 
 
+def result_buffers(n, L=0, want_fit=False, alloc=np.empty):
+    """A result set for n pixels (the `out` of Context.invert and
+    Server.submit): model [n, 11] f64, chi2 f32, iters and accepted i32, and
+    with want_fit the fitted profiles [n, 4, L] f32.  alloc(shape, dtype)
+    makes each array (e.g. one that returns pinned host memory)."""
+    return dict(model=alloc((n, NPAR), np.float64), chi2=alloc((n,), np.float32),
+                iters=alloc((n,), np.int32), accepted=alloc((n,), np.int32),
+                fit=alloc((n, 4, L), np.float32) if want_fit else None)
+
+
+def _result_set(n, L, want_fit, out):
+    """out checked against result_buffers(n, L, want_fit) (ValueError if an
+    array does not match), or a fresh zeroed set when out is None."""
+    if out is None:
+        return result_buffers(n, L, want_fit, np.zeros)
+    want = result_buffers(n, L, want_fit, lambda sh, dt: (sh, dt))
+    for k, v in want.items():
+        a = out.get(k)
+        if v is None:
+            continue
+        if (not isinstance(a, np.ndarray) or a.shape != v[0] or a.dtype != v[1]
+                or not a.flags.c_contiguous or not a.flags.writeable):
+            raise ValueError("out[%r]: need a writable C-contiguous %s array of shape %s"
+                             % (k, np.dtype(v[1]).name, v[0]))
+    return out if want_fit else dict(out, fit=None)
+
+
 class Server:
     """Serving loop on the device (pm_server): scans are submitted as they
     arrive, share the GPU, and are collected with wait()."""
@@ -281,15 +306,7 @@ class Server:
         self.h = h
         self._pending = {}
 
-    @staticmethod
-    def result_buffers(n, L=0, want_fit=False, alloc=np.empty):
-        """A result set for a scan of n pixels (the `out` of submit()):
-        model [n, 11] f64, chi2 f32, iters and accepted i32, and with
-        want_fit the fitted profiles [n, 4, L] f32.  alloc(shape, dtype)
-        makes each array (e.g. one that returns pinned host memory)."""
-        return dict(model=alloc((n, NPAR), np.float64), chi2=alloc((n,), np.float32),
-                    iters=alloc((n,), np.int32), accepted=alloc((n,), np.int32),
-                    fit=alloc((n, 4, L), np.float32) if want_fit else None)
+    result_buffers = staticmethod(lambda *a, **k: result_buffers(*a, **k))
 
     def submit(self, ctx, stokes, slight=None, want_fit=False, out=None):
         """Queues a scan; returns a ticket.  The results go into `out` (a
@@ -299,20 +316,7 @@ class Server:
         st = np.ascontiguousarray(stokes, dtype=np.float32)
         n = st.shape[0]
         sl = None if slight is None else np.ascontiguousarray(slight, dtype=np.float32)
-        if out is None:
-            out = self.result_buffers(n, ctx.L, want_fit, np.zeros)
-        else:
-            want = self.result_buffers(n, ctx.L, want_fit, lambda sh, dt: (sh, dt))
-            for k, v in want.items():
-                a = out.get(k)
-                if v is None:
-                    continue
-                if (not isinstance(a, np.ndarray) or a.shape != v[0] or a.dtype != v[1]
-                        or not a.flags.c_contiguous or not a.flags.writeable):
-                    raise ValueError("out[%r]: need a writable C-contiguous %s array of shape %s"
-                                     % (k, np.dtype(v[1]).name, v[0]))
-            if not want_fit:
-                out = dict(out, fit=None)
+        out = _result_set(n, ctx.L, want_fit, out)
         t = C.c_longlong(0)
         _check(lib().pm_submit(self.h, ctx.h, st.ctypes.data, _ptr(sl), n,
                                out["model"].ctypes.data, out["chi2"].ctypes.data,

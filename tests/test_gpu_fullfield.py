@@ -96,6 +96,20 @@ def test_pixels_across_the_field_match_the_oracle(field, fp64):
     assert np.all(f >= cases.FRAC), f
 
 
+def test_host_buffers_give_the_device_bits(field, fp64):
+    """pm_invert on host buffers (the field in chunks over the context's
+    streams, uploads and copy-backs overlapping other chunks' kernels),
+    into pinned and into pageable result arrays: the device-pointer
+    path's bits on every pixel."""
+    from pmilos.serving import _pinned
+    host = field["stokes"].cpu().numpy()
+    for alloc in (_pinned, np.empty):
+        say("case full field host buffers, %s results" % alloc.__name__)
+        got = field["ctx"].invert(host, out=pmilos.result_buffers(NPIX, alloc=alloc))
+        for k in fp64:
+            assert np.array_equal(got[k], fp64[k]), k
+
+
 def test_bits_do_not_depend_on_order_or_workgroup(field, fp64):
     """A second pass with the other workgroup size, and a pass over the
     field with its pixels in another order: the same bits per pixel."""
