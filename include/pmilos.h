@@ -132,7 +132,7 @@ int pm_synth_device(pm_ctx *ctx, const double *d_models, int n, double *d_spectr
  * lm_mils returns -1 for nfree == 0, as milos.c:997 leaves it); accepted
  * nullable [npix] int (chi2-decreasing trips); fit nullable [npix][4][L] float
  * (the last trial synthesis, milosUtils.c:569).  Host pointers, synchronous:
- * from 2^18 pixels on, the pixels go in three chunks (1/16, 14/16, 1/16) on
+ * from 2^20 pixels on, the pixels go in three chunks (1/16, 14/16, 1/16) on
  * three streams of the context, each chunk's upload and copy-back
  * overlapping the other chunks' kernels (pinned host buffers copy
  * fastest), with the same results as one pm_invert_device call.  The device buffers of the host-pointer calls (pm_synth,

@@ -31,8 +31,9 @@
 // back.  Equal chunks do worse: every chunk boundary leaves part of the GPU
 // idle while a persistent kernel drains (1024^2 pixels x 30 samples, FP64,
 // pinned buffers: one chunk 161.6 ms, eight equal ones 162.3 ms, 1/16 +
-// 15/16 153.1 ms, 1/16 + 14/16 + 1/16 154.7 ms).
-constexpr int PM_HOST_STREAMS = 3, PM_HOST_CHUNKS = 3, PM_HOST_SPLIT_MIN = 1 << 18;
+// 15/16 153.1 ms, 1/16 + 14/16 + 1/16 154.7 ms).  Below 2^20 pixels the
+// split costs more than it hides (262,144 pixels at 6 samples: 12% slower).
+constexpr int PM_HOST_STREAMS = 3, PM_HOST_CHUNKS = 3, PM_HOST_SPLIT_MIN = 1 << 20;
 
 struct pm_ctx {
     PmConst h;

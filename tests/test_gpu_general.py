@@ -189,10 +189,10 @@ def test_invert_per_pixel_stray_light():
     a = ctx.invert(c.cube[:256], slight=rep[:256])
     b = ctx.invert(c.cube[:256])
     assert np.array_equal(a["model"], b["model"]) and np.array_equal(a["iters"], b["iters"])
-    # host buffers of more than 2^18 pixels (pm_invert_sl splits them over
+    # host buffers of more than 2^20 pixels (pm_invert_sl splits them over
     # its streams): the cube tiled gives the untiled results tiled, fit
     # included
-    reps = -(-(1 << 18) // len(c.cube)) + 1
+    reps = -(-(1 << 20) // len(c.cube)) + 1
     say("case per-pixel stray light, %d pixels in chunks" % (reps * len(c.cube)))
     one = ctx.invert(c.cube, slight=c.slight, want_fit=True)
     big = ctx.invert(np.tile(c.cube, (reps, 1, 1)), slight=np.tile(c.slight, (reps, 1, 1)),
