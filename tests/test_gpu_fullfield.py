@@ -108,6 +108,15 @@ def test_host_buffers_give_the_device_bits(field, fp64):
         got = field["ctx"].invert(host, out=pmilos.result_buffers(NPIX, alloc=alloc))
         for k in fp64:
             assert np.array_equal(got[k], fp64[k]), k
+    # through the C ABI with the optional outputs left out, as a C caller
+    # of pm_invert does
+    say("case full field host buffers, no accepted / fit")
+    o = pmilos.result_buffers(NPIX)
+    rc = pmilos.lib().pm_invert(field["ctx"].h, host.ctypes.data, NPIX, o["model"].ctypes.data,
+                                o["chi2"].ctypes.data, o["iters"].ctypes.data, None, None)
+    assert rc == 0, pmilos.lib().pm_strerror(rc)
+    for k in ("model", "chi2", "iters"):
+        assert np.array_equal(o[k], fp64[k]), k
 
 
 def test_bits_do_not_depend_on_order_or_workgroup(field, fp64):
