@@ -224,7 +224,12 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
     }
     group_sync<W>();
     {
-        const int r = t / NB, b = t % NB;
+        // lane t takes block t / R of row t % R: the lanes that read LDS in
+        // the same cycle (16 for 8-byte, 32 for 4-byte reads) spread over
+        // the rows, whose odd stride puts them in different banks, instead
+        // of all taking blocks of one row B elements apart (two banks for
+        // B = 8 doubles)
+        const int r = t % R, b = t / R;
         const T *src = pad + r * RS + half - HC + b * B;
         T w[CF::NWIN];
 #pragma unroll
@@ -355,7 +360,8 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
         }
     }
     group_sync<W>();
-    const int r = t / NB, b = t % NB, m0 = b * B;
+    // lane t takes block t / R of row t % R (bank spread, as in convF)
+    const int r = t % R, b = t / R, m0 = b * B;
     A acc[B];
 #pragma unroll
     for (int o = 0; o < B; o++) acc[o] = 0;
