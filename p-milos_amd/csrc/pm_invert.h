@@ -951,7 +951,32 @@ pm_invert_kernel(InvArgs a)
             // covar = alpha with the diagonal scaled by (1 + lambda), :559-563
             double delta[NT];
             const bool check = !(chk_lambda >= 0.0 && flambda >= chk_lambda);
-            bool ok = ldl_solve<NT>(ab, 1.0 + flambda, delta, check);
+            bool ok = false;
+            if constexpr (W > 64) {
+                // a pixel over two waves: the first solves, the second takes
+                // its step and verdict from LDS (the slots of the J^T J
+                // cross-wave sum, past the row buffer, idle here) instead of
+                // repeating the solve
+                static_assert((LD::PAD_BYTES + 7) / 8 <= (LD::CH + 1) * W &&
+                              NT + 1 <= LD::CH, "solve exchange slots past the row buffer");
+                double *xs = red + (LD::CH + 1) * W;
+                if (t < 64) {
+                    ok = ldl_solve<NT>(ab, 1.0 + flambda, delta, check);
+                    if (t == 0) {
+#pragma unroll
+                        for (int i = 0; i < NT; i++) xs[i] = delta[i];
+                        xs[NT] = ok ? 1.0 : 0.0;
+                    }
+                }
+                group_sync<W>();
+                if (t >= 64) {
+#pragma unroll
+                    for (int i = 0; i < NT; i++) delta[i] = xs[i];
+                    ok = xs[NT] != 0.0;
+                }
+            } else {
+                ok = ldl_solve<NT>(ab, 1.0 + flambda, delta, check);
+            }
             if (ok && check) chk_lambda = flambda;
             if (!ok) {
                 double *ws = (double *)pad;
