@@ -316,7 +316,8 @@ __device__ __forceinline__ void convC(T *pad, HP h, int jlo, int jhi, int t, int
 // samples of output m under tap j sit at TC + m - j + L and each lane's
 // block of B consecutive outputs reads one contiguous window per chunk of
 // TC taps.  The taps come from the pixel's kernel in LDS, in ascending order
-// as in convC (chunks padded with zero taps), so the results are the same.
+// as in convC; the last chunk reads past the window's end, where the kernel
+// holds exact zeros (mac_kernels), so the results are the same.
 // Every lane loads its windows before any lane stores; the doubled rows are
 // cleared afterwards (the other convolutions expect zero margins).
 template <typename T, int W, int NLPL, int R>
@@ -374,8 +375,7 @@ __device__ __forceinline__ void convCF(T *pad, const T *h, int jlo, int jhi, int
             for (int i = 0; i < CF::NWIN; i++) w[i] = src[i];
 #pragma unroll
             for (int u = 0; u < TC; u++) {
-                const int j = j0 + u;
-                const A hj = j < jhi ? (A)h[j < jhi ? j : jhi - 1] : A(0);
+                const A hj = (A)h[j0 + u];
 #pragma unroll
                 for (int o = 0; o < B; o++) acc[o] += hj * (A)w[o + TC - 1 - u];
             }
@@ -509,13 +509,18 @@ __device__ __forceinline__ void mac_kernels(PmC c, const double *m,
         x[1][k] = T(g / MC * (a2[k] - A(1)));
     }
     if (c->use_psf) convPSF<T, W, NLPL, 2, true, false>(c, pad, t, L, x);
+    tap_window<T, W, NLPL>(x[0], t, L, tp.mlo, tp.mhi);
+    tap_window<T, W, NLPL>(x[1], t, L, tp.dlo, tp.dhi);
+    // every slot of gm / gd, with exact zeros outside the tap windows: the
+    // register-blocked convolution reads whole chunks of taps past a
+    // window's end without a test (the convolutions use no tap outside
+    // their window)
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
-        if (lam < L) { gm[lam] = x[0][k]; gd[lam] = x[1][k]; }
+        gm[lam] = (lam >= tp.mlo && lam < tp.mhi) ? x[0][k] : T(0);
+        gd[lam] = (lam >= tp.dlo && lam < tp.dhi) ? x[1][k] : T(0);
     }
-    tap_window<T, W, NLPL>(x[0], t, L, tp.mlo, tp.mhi);
-    tap_window<T, W, NLPL>(x[1], t, L, tp.dlo, tp.dhi);
     double tsum = 0;
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
