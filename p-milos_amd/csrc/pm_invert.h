@@ -250,18 +250,27 @@ __device__ __forceinline__ void convF(PmC c, T *pad, int t, int L,
         group_sync<W>();                       // every window loaded before any store
         const bool irow = IC && (r % 4) == 0;
         const T icr = (R == 8 && r == 4) ? ic[1] : ic[0];
+        // one FMA per output: icr - acc on the I rows (the continuum trick,
+        // rounded once as the subtraction is), acc + (-0) elsewhere (acc's
+        // own bits, signed zeros included)
+        const A sg = irow ? A(-1) : A(1), off = irow ? (A)icr : A(-0.0);
         T *dst = pad + r * RS + half + b * B;
+        // every output is stored; those at or past L land in the row's
+        // zero margin and are cleared again below
 #pragma unroll
-        for (int o = 0; o < B; o++)
-            if (b * B + o < L) dst[o] = irow ? T((A)icr - acc[o]) : T(acc[o]);
+        for (int o = 0; o < B; o++) dst[o] = T(fma(acc[o], sg, off));
     }
     group_sync<W>();
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {
         int lam = t + k * W;
-        if (lam < L)
+        if (lam < L) {
 #pragma unroll
             for (int r = 0; r < R; r++) x[r][k] = pad[r * RS + half + lam];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; r++) pad[r * RS + half + lam] = T(0);
+        }
     }
     group_sync<W>();
 }
