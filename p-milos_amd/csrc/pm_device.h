@@ -51,8 +51,9 @@ struct PmConst {
     double lam[PM_LMAX];
     double dlam[PM_LMAX];   // lam - lambda0 in double (mil_sinrf.c:124 subtracts in double)
     double G[PM_LMAX];
-    // PSF taps [glo, ghi) whose weight reaches 1e-17 (FP64 form) / 1e-9
-    // (FP32 form) of the peak: the others are below the rounding of the sum
+    // PSF taps [glo, ghi) whose weight reaches 1e-16 (FP64 form) / 1e-8
+    // (FP32 form) of the peak, below each form's unit roundoff: the others
+    // are below the rounding of the sum
     int glo64, ghi64, glo32, ghi32;
     // the same taps centred on the kernel centre: gcNN[PM_HC + d] = G[L/2 + d]
     // inside the form's tap range, 0 outside; reachNN = the largest |d| with

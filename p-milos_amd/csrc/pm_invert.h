@@ -151,12 +151,13 @@ __device__ __forceinline__ void convR(PmC c, T *pad, int t, int L,
 
 // Half-width of the register-blocked correlation for rows of NX samples
 // (the window start half - HC stays inside the row buffer for L/2 >= HC):
-// 11 taps each side in the FP64 form (taps down to 1e-17 of the peak), 8 in
-// the FP32 form (down to 1e-9).  That covers Gaussian PSFs narrower than
-// 3.19 (FP64) and 3.29 (FP32) samples FWHM; a 60 mA Gaussian on a 20 mA grid
-// is 3.0.  Wider PSFs take the tap loop (convR), with the same sums.
+// 10 taps each side in the FP64 form (taps down to 1e-16 of the peak, below
+// its unit roundoff), 7 in the FP32 form (down to 1e-8).  That covers
+// Gaussian PSFs narrower than 3.01 (FP64) and 3.10 (FP32) samples FWHM; a
+// 60 mA Gaussian on a 20 mA grid is 3.0.  Wider PSFs take the tap loop
+// (convR), with the same sums.
 template <typename T, int NX> struct ConvHC {
-    static constexpr int cap = sizeof(T) == 8 ? PM_HC : 8;
+    static constexpr int cap = sizeof(T) == 8 ? 10 : 7;
     static constexpr int value = NX / 2 - 1 < cap ? NX / 2 - 1 : cap;
 };
 
@@ -451,8 +452,9 @@ __device__ __forceinline__ void convPSF(PmC c, T *pad, int t, int L,
 }
 
 // Tap windows of the two macroturbulence kernels: [lo, hi) holds every tap
-// whose magnitude reaches 1e-17 (FP64 form) / 1e-9 (FP32 form) of the
-// kernel's largest; the rest are below the rounding of the sums they enter.
+// whose magnitude reaches 1e-16 (FP64 form) / 1e-8 (FP32 form) of the
+// kernel's largest, below each form's unit roundoff; the rest are below the
+// rounding of the sums they enter.
 struct Taps {
     int mlo, mhi, dlo, dhi;
     double msum;         // sum of the kernel's taps in [mlo, mhi)
@@ -466,7 +468,7 @@ __device__ __forceinline__ void tap_window(const T v[NLPL], int t, int L, int &l
     for (int k = 0; k < NLPL; k++)
         if (t + k * W < L) mx = fmax(mx, fabs((double)v[k]));
     mx = group_max<W>(mx);
-    const double thr = mx * (sizeof(T) == 8 ? 1e-17 : 1e-9);
+    const double thr = mx * (sizeof(T) == 8 ? 1e-16 : 1e-8);
     int a = L, b = 0;
 #pragma unroll
     for (int k = 0; k < NLPL; k++) {

@@ -138,8 +138,8 @@ static inline int pm_const_build(const pm_config *cfg, PmConst &c)
         int lo64 = L, hi64 = 0, lo32 = L, hi32 = 0;
         for (int i = 0; i < L; i++) {
             double r = gmax > 0 ? fabs(c.G[i]) / gmax : 0;
-            if (r >= 1e-17) { lo64 = lo64 < i ? lo64 : i; hi64 = i + 1; }
-            if (r >= 1e-9) { lo32 = lo32 < i ? lo32 : i; hi32 = i + 1; }
+            if (r >= 1e-16) { lo64 = lo64 < i ? lo64 : i; hi64 = i + 1; }
+            if (r >= 1e-8) { lo32 = lo32 < i ? lo32 : i; hi32 = i + 1; }
         }
         if (!c.use_psf) { lo64 = lo32 = 0; hi64 = hi32 = 0; }
         c.glo64 = lo64; c.ghi64 = hi64; c.glo32 = lo32; c.ghi32 = hi32;
