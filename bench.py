@@ -51,12 +51,12 @@ PEAK_FP32_TFLOPS = 157.3
 # 1048576 pixels (FETCH_SIZE scaled by 2.0 and WRITE_SIZE by 1.0, the factors
 # a plain 512 MiB streaming kernel reads in the same runs); the summaries
 # are profiles/pm_invert_kernel_l30_1048576px_hbm.md
-TRAFFIC_B_PER_PX = {"fp64": 596.64, "mixed": 595.66}
+TRAFFIC_B_PER_PX = {"fp64": 591.63, "mixed": 595.67}
 # lane FLOPs issued per pixel by the same launches (64 x the
 # SQ_INSTS_VALU_FLOPS_FP64 / _FP32 counters: every lane slot of an issued
 # instruction); the summary is profiles/pm_invert_kernel_l30_1048576px_valu.md
-ISSUED_FLOP_PER_PX = {"fp64": {"fp64": 4719359, "fp32": 315},
-                      "mixed": {"fp64": 570306, "fp32": 2839929}}
+ISSUED_FLOP_PER_PX = {"fp64": {"fp64": 4599999, "fp32": 315},
+                      "mixed": {"fp64": 570461, "fp32": 2725551}}
 ALGO_B_PER_PX = 4 * 30 * 4 + 11 * 8 + 3 * 4     # Stokes in; model, chi2, iters, accepted out
 
 
