@@ -110,16 +110,17 @@ def scan_cubes(ctxs, kinds, distinct_small=4, distinct_large=2):
 
 def calibrate(ctxs, made):
     """Pixels/s of each context alone on the first cube of its largest kind
-    (a bounded slice), used to size the arrival rate for a target
-    utilisation."""
+    (a bounded slice, results into a pinned set as in the run), used to size
+    the arrival rate for a target utilisation."""
     rate = {}
     for L, ctx in ctxs.items():
         size = max(s for s, l in made if l == L)
         cube = made[(size, L)][0]
         n = min(len(cube), 262144 if L <= 30 else 65536)
-        ctx.invert(cube[:4096])
+        out = pmilos.result_buffers(n, alloc=_pinned)
+        ctx.invert(cube[:n], out=out)
         t = time.perf_counter()
-        ctx.invert(cube[:n])
+        ctx.invert(cube[:n], out=out)
         rate[L] = n / (time.perf_counter() - t)
     return rate
 
