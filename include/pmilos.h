@@ -304,7 +304,8 @@ int pm_poll(pm_server *srv, long long ticket);
 
 /* Blocks until the scan is done and retires the ticket; latency_s
  * (nullable) = submission to results in host memory, in seconds.  Returns
- * the scan's status. */
+ * the scan's status; PM_EINVAL for a ticket already retired, also when
+ * another thread's pm_wait retired it while this one waited. */
 int pm_wait(pm_server *srv, long long ticket, double *latency_s);
 
 #ifdef __cplusplus

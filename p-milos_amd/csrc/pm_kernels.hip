@@ -768,15 +768,23 @@ int host_setup(pm_ctx *ctx)
     auto &h = ctx->host;
     if (h.counters) return PM_OK;
     for (auto &st : h.s)
-        if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess)
+        if (!st && hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+            st = nullptr;
             return PM_EDEVICE;
-    if (hipMalloc(&h.fallbacks, sizeof(unsigned long long) * PM_HOST_CHUNKS) != hipSuccess)
-        return PM_ENOMEM;
-    if (!h.cleared && hipEventCreateWithFlags(&h.cleared, hipEventDisableTiming) != hipSuccess)
-        return PM_EDEVICE;
-    if (hipMalloc(&h.counters, pm_counters_bytes() * PM_HOST_CHUNKS) != hipSuccess) {
-        (void)hipFree(h.fallbacks);
+        }
+    // each step is skipped when an earlier call that failed later already
+    // made it, so a retry after an error leaks nothing
+    if (!h.fallbacks &&
+        hipMalloc(&h.fallbacks, sizeof(unsigned long long) * PM_HOST_CHUNKS) != hipSuccess) {
         h.fallbacks = nullptr;
+        return PM_ENOMEM;
+    }
+    if (!h.cleared && hipEventCreateWithFlags(&h.cleared, hipEventDisableTiming) != hipSuccess) {
+        h.cleared = nullptr;
+        return PM_EDEVICE;
+    }
+    if (hipMalloc(&h.counters, pm_counters_bytes() * PM_HOST_CHUNKS) != hipSuccess) {
+        h.counters = nullptr;
         return PM_ENOMEM;
     }
     return PM_OK;

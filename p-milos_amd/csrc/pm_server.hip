@@ -60,37 +60,50 @@ size_t up(size_t n, size_t a) { return (n + a - 1) / a * a; }
 
 // Device blocks are kept for reuse until the server ends: freeing device
 // memory waits for the whole device, which would hold up every other scan.
+// get() runs on the admitter thread outside the server's lock (a hipMalloc
+// can take milliseconds) and put() on the copier thread under it, so the
+// block list has a lock of its own; a new block is allocated outside it.
 class DevPool {
     struct Block { char *p; size_t n; bool used; };
     std::vector<Block> blocks_;
+    std::mutex mu_;
 
 public:
     char *get(size_t n)
     {
         n = up(n, size_t(2) << 20);
-        Block *best = nullptr;
-        for (auto &b : blocks_)
-            if (!b.used && b.n >= n && b.n <= 2 * n && (!best || b.n < best->n)) best = &b;
-        if (best) { best->used = true; return best->p; }
-        char *p = nullptr;
-        if (hipMalloc(&p, n) != hipSuccess) {
-            // give back idle blocks and try once more
-            for (auto it = blocks_.begin(); it != blocks_.end();) {
-                if (!it->used) { (void)hipFree(it->p); it = blocks_.erase(it); }
-                else ++it;
-            }
-            if (hipMalloc(&p, n) != hipSuccess) return nullptr;
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            Block *best = nullptr;
+            for (auto &b : blocks_)
+                if (!b.used && b.n >= n && b.n <= 2 * n && (!best || b.n < best->n)) best = &b;
+            if (best) { best->used = true; return best->p; }
         }
+        char *p = nullptr;
+        if (hipMalloc(&p, n) == hipSuccess) {
+            std::lock_guard<std::mutex> g(mu_);
+            blocks_.push_back({p, n, true});
+            return p;
+        }
+        // give back idle blocks and try once more
+        std::lock_guard<std::mutex> g(mu_);
+        for (auto it = blocks_.begin(); it != blocks_.end();) {
+            if (!it->used) { (void)hipFree(it->p); it = blocks_.erase(it); }
+            else ++it;
+        }
+        if (hipMalloc(&p, n) != hipSuccess) return nullptr;
         blocks_.push_back({p, n, true});
         return p;
     }
     void put(char *p)
     {
+        std::lock_guard<std::mutex> g(mu_);
         for (auto &b : blocks_)
             if (b.p == p) b.used = false;
     }
     void release()
     {
+        std::lock_guard<std::mutex> g(mu_);
         for (auto &b : blocks_) (void)hipFree(b.p);
         blocks_.clear();
     }
@@ -538,10 +551,15 @@ extern "C" int pm_wait(pm_server *s, long long ticket, double *latency_s)
 {
     if (!s) return PM_EINVAL;
     std::unique_lock<std::mutex> lk(s->mu);
+    // the ticket is looked up again on every wake-up: another thread's
+    // pm_wait on the same ticket may have collected and freed it meanwhile
     auto it = s->all.find(ticket);
+    s->cv.wait(lk, [&] {
+        it = s->all.find(ticket);
+        return it == s->all.end() || it->second->state == DONE;
+    });
     if (it == s->all.end()) return PM_EINVAL;
     Req *r = it->second;
-    s->cv.wait(lk, [&] { return r->state == DONE; });
     if (latency_s)
         *latency_s = std::chrono::duration<double>(r->t_done - r->t_submit).count();
     int rc = r->rc;

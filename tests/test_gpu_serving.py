@@ -1,8 +1,11 @@
 """GPU tests of the serving loop (pm_server_*, pm_submit, pm_poll, pm_wait):
 scans of different sizes and samplings submitted together give exactly the
 bits each gives alone through pm_invert; admission respects the HBM budget;
-a small scan submitted behind a large one finishes first."""
+two waits on one ticket retire it once; a small scan submitted behind a
+large one finishes first."""
+import ctypes
 import sys
+import threading
 
 import numpy as np
 import pytest
@@ -118,6 +121,35 @@ def test_budget_admission_and_errors():
         assert out["model"].shape == (0, 11)
         with pytest.raises(pmilos.PmError):
             srv.poll(t)
+    finally:
+        srv.close()
+
+
+def test_two_waits_on_one_ticket():
+    """Two threads in pm_wait on one ticket: one collects the scan with its
+    results, the other gets PM_EINVAL (-22) instead of the freed request."""
+    say("case serving two waits on one ticket")
+    ctx = _ctx(30)
+    q = _cube(ctx, 20000, 210)
+    alone = ctx.invert(q)
+    srv = pmilos.Server()
+    try:
+        t = srv.submit(ctx, q)
+        rcs = []
+
+        def wait_raw():
+            lat = ctypes.c_double(0)
+            rcs.append(pmilos.lib().pm_wait(srv.h, t, ctypes.byref(lat)))
+
+        th = [threading.Thread(target=wait_raw) for _ in range(2)]
+        for x in th:
+            x.start()
+        for x in th:
+            x.join(timeout=60)
+        assert not any(x.is_alive() for x in th)
+        assert sorted(rcs) == [-22, 0]
+        _, _, out = srv._pending.pop(t)
+        _same(out, alone)
     finally:
         srv.close()
 
