@@ -279,7 +279,10 @@ void pm_graph_destroy(pm_graph *g);
  * back into the caller's buffers.  A scan's results are the same bits as pm_invert / pm_invert_sl
  * give for it alone.  The caller keeps its buffers alive and unchanged
  * until pm_wait returns for the ticket.  One host thread may submit while
- * others wait. */
+ * others wait.  While a context has scans queued or running, the server's
+ * threads use it: make no other call on that context (pm_submit aside)
+ * until pm_wait has returned for each of them, and use a second context
+ * for other work meanwhile. */
 typedef struct pm_server pm_server;
 
 typedef struct {
