@@ -329,12 +329,15 @@ class Server:
         return _check(lib().pm_poll(self.h, ticket)) == 1
 
     def wait(self, ticket):
-        """Blocks until the scan is done: (results dict, latency in s)."""
+        """Blocks until the scan is done: (results dict, latency in s).
+        PmError for a ticket already collected."""
         lat = C.c_double(0)
         rc = lib().pm_wait(self.h, ticket, C.byref(lat))
-        _, _, out = self._pending.pop(ticket)
+        entry = self._pending.pop(ticket, None)
         _check(rc)
-        return out, lat.value
+        if entry is None:
+            raise PmError("ticket %d has no result set here" % ticket)
+        return entry[2], lat.value
 
     def close(self):
         if getattr(self, "h", None):

@@ -150,6 +150,9 @@ def test_two_waits_on_one_ticket():
         assert sorted(rcs) == [-22, 0]
         _, _, out = srv._pending.pop(t)
         _same(out, alone)
+        # the binding's own wait on the retired ticket raises PmError
+        with pytest.raises(pmilos.PmError):
+            srv.wait(t)
     finally:
         srv.close()
 
